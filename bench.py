@@ -1,0 +1,184 @@
+"""Benchmark: SPH dam break, 1,048,576 fluid particles, one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dam_break_1m|dam_break_32k]
+                    [--warm-start zero|prev] [--no-cpu-baseline]
+
+Prints ONE JSON line. A step = one SolverSPH.step (hash/sort, neighbour lists, density,
+advection, IISPH pressure solve to 0.1 % mean compression, integration) over the whole
+scene; value = fluid particle-updates/s over the K timed steps (inputs resident in HBM).
+roofline: the kernel class with the largest GPU time, its algorithmic bytes per launch
+(compulsory unique bytes, each array once, neighbour re-reads not counted, list bytes
+counted) divided by its mean launch time from HIP events recorded on the solver's stream
+during the timed steps. cpu_baseline: the same-order FP32 CPU restatement (oracle) of the
+same step, OpenMP over the host cores, on a bounded sample of steps.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nbar: float) -> dict[str, float]:
+    """Compulsory bytes per fluid particle and launch, per kernel class (nbar = mean
+    fluid-fluid list length; 4 B per list entry + 4 B count)."""
+    L = 4.0 * nbar + 4.0
+    return {
+        "build": 28 + 3 * 20 + 92 + 12,
+        "neighbors": 16 + L,
+        "density": 16 + 24 + L,
+        "advect": 16 + 16 + 4 + 32 + L,
+        "source": 16 + 16 + 16 + 4 + 4 + 12 + L,
+        "pressure_acc": 16 + 4 + 16 + 16 + L,
+        "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 8 + L,
+        "reduce": 0.0,
+        "final": 16 + 4 + 16 + 16 + 16 + 12 + 4 + 40 + L,
+        "passthrough": 0.0,
+    }
+
+
+def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
+    from oracle import oracle as O
+
+    pb = O.problem_from_model(model, solver)
+    nthreads = os.cpu_count() or 1
+    nt = min(nthreads, 16)
+    q, qd, pp = pb.q0, pb.qd0, None
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        r = O.step(pb, q, qd, pprev=pp, dt=dt, precision="f32", nthreads=nt)
+        q, qd, pp = r["q"], r["qd"], r["pprev"]
+        steps += 1
+        if time.perf_counter() - t0 > budget_s / 2 or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    nf = solver.fluid_count
+    return {
+        "value": nf * steps / el,
+        "unit": "particle-updates/s",
+        "cores": nt,
+        "kind": "port",
+        "sample": f"{steps} steps of the same workload from the initial state, same-order FP32 C "
+                  f"restatement (OpenMP {nt} threads, {platform.processor() or platform.machine()})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="dam_break_1m")
+    ap.add_argument("--warm-start", default="prev")
+    ap.add_argument("--check-every", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from newton_sph_dev_amd import SolverSPH, _native as N, scenes
+
+    cfg = {"dam_break_1m": scenes.DAM_BREAK_1M, "dam_break_32k": scenes.DAM_BREAK_32K}[args.workload]
+    print(f"[bench] building {cfg.name}", file=sys.stderr, flush=True)
+    model = scenes.build_dam_break(cfg).finalize("cuda:0")
+    solver = SolverSPH(model, cfg.support, warm_start=args.warm_start, check_every=args.check_every)
+    s0, s1 = model.state(), model.state()
+    q0 = model.particle_q.clone()
+    qd0 = model.particle_qd.clone()
+    dt = scenes.DT
+    print(f"[bench] fluid={solver.fluid_count} boundary={solver.boundary_count}; warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        solver.step(s0, s1, None, None, dt)
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+
+    lib = N.load()
+    N.call("nsph_solver_set_timing", solver._handle, 1)
+    iters = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        solver.step(s0, s1, None, None, dt)
+        iters.append(solver.last_report["max_iterations"])
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nk = lib.nsph_kernel_count()
+    ms = (C.c_double * nk)()
+    cnt = (C.c_int * nk)()
+    N.call("nsph_solver_timing", solver._handle, ms, cnt)
+    N.call("nsph_solver_set_timing", solver._handle, 0)
+    names = [lib.nsph_kernel_name(k).decode() for k in range(nk)]
+    per = {names[k]: (ms[k], cnt[k]) for k in range(nk)}
+    counts = solver.fields()["neighbor_count"]
+    nf = solver.fluid_count
+    nbar = float(counts.sum().item()) / max(nf, 1)
+    ab = algorithmic_bytes(nbar)
+    dom = max((k for k in names if ab[k] > 0), key=lambda k: per[k][0])
+    avg_s = per[dom][0] / max(per[dom][1], 1) / 1e3
+    achieved = ab[dom] * nf / avg_s / 1e9
+    value = nf * args.steps / el
+    kern = {k: {"ms_total": round(per[k][0], 3), "launches": per[k][1],
+                "ms_avg": round(per[k][0] / max(per[k][1], 1), 4),
+                "GBps": round(ab[k] * nf / (per[k][0] / max(per[k][1], 1) / 1e3) / 1e9, 1) if ab[k] > 0 and per[k][1] else None}
+            for k in names}
+    step_bytes = sum(ab[k] * nf * per[k][1] for k in names) / args.steps
+    out = {
+        "metric": "particle-updates/s (1M SPH) + % HBM peak",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded dam-break lattice)",
+        "config": {
+            "workload": cfg.name,
+            "fluid_particles": nf,
+            "boundary_particles": solver.boundary_count,
+            "support_radius": cfg.support,
+            "dt": dt,
+            "tolerance": solver.tolerance,
+            "warm_start": args.warm_start,
+            "mean_iterations": float(np.mean(iters)),
+            "mean_fluid_neighbors": nbar,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": ab[dom] * nf,
+        },
+        "step_hbm_frac": step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS,
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(model, solver, q0, qd0, dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * nsph.h -- C ABI of libnsph.so, the gfx950 (MI355X) SPH particle step.
+ *
+ * Every entry point returns an int status (NSPH_OK == 0).  On failure the
+ * message is available from nsph_last_error().  No C++ exception crosses
+ * this boundary.  Pointers named d_* are device pointers (HBM) owned by the
+ * caller; the library owns only its scratch, allocated in the *_create and
+ * *_reserve calls and never inside nsph_solver_step / nsph_grid_build after
+ * a matching reserve.  All work is enqueued on the context's stream.
+ *
+ * Reference interfaces replaced (paths relative to the Newton source tree):
+ *   HashGrid      -> warp.HashGrid as created in newton/_src/sim/builder.py:9851-9855,
+ *                    reserved in newton/_src/solvers/xpbd/solver_xpbd.py:136-139,
+ *                    built in solver_xpbd.py:315-319, consumed through
+ *                    hash_grid_point_id/hash_grid_query/hash_grid_query_next in
+ *                    newton/_src/solvers/semi_implicit/kernels_contact.py:54-73 and
+ *                    newton/_src/solvers/xpbd/kernels.py:244-264.
+ *   Solver step   -> SolverBase.step(state_in, state_out, control, contacts, dt)
+ *                    (newton/_src/solvers/solver.py:301-316) with the particle
+ *                    integrator of solver.py:10-47 (integrate_particles).
+ */
+#ifndef NSPH_H
+#define NSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSPH_ABI_VERSION 1
+
+enum nsph_status {
+    NSPH_OK = 0,
+    NSPH_ERR_INVALID = 1,       /* bad argument or handle                              */
+    NSPH_ERR_HIP = 2,           /* HIP runtime error                                   */
+    NSPH_ERR_CAPACITY = 3,      /* n above the reserved capacity / neighbour overflow  */
+    NSPH_ERR_NOT_BUILT = 4,     /* grid queried before build                           */
+    NSPH_ERR_NONFINITE = 5,     /* non-finite value in the step output                 */
+    NSPH_ERR_NOT_CONVERGED = 6  /* pressure solve hit max_iters in at least one world  */
+};
+
+/* Particle flag bits, same values as ParticleFlags (newton/_src/geometry/flags.py:8-14). */
+#define NSPH_PARTICLE_ACTIVE 1
+
+typedef struct nsph_ctx nsph_ctx;
+typedef struct nsph_grid nsph_grid;
+typedef struct nsph_solver nsph_solver;
+
+/* ---- context --------------------------------------------------------------------- */
+
+/* device: HIP ordinal; stream: hipStream_t (NULL = default stream). */
+int nsph_ctx_create(int device, void* stream, nsph_ctx** out);
+int nsph_ctx_set_stream(nsph_ctx* ctx, void* stream);
+int nsph_ctx_synchronize(nsph_ctx* ctx);
+int nsph_ctx_destroy(nsph_ctx* ctx);
+/* Last error message of the calling thread ("" if none). */
+const char* nsph_last_error(void);
+int nsph_abi_version(void);
+
+/* ---- HashGrid (replaces warp.HashGrid; builder.py:9853, solver_xpbd.py:139,319) ---- */
+
+/* A periodic hash grid of dim_x*dim_y*dim_z cells: cell = floor(x / radius) mod dim,
+ * as the reference's HashGrid(128,128,128).  Each dim must be >= 3. */
+int nsph_grid_create(nsph_ctx* ctx, int dim_x, int dim_y, int dim_z, nsph_grid** out);
+/* Pre-allocate for n points (solver_xpbd.py:139 particle_grid.reserve). */
+int nsph_grid_reserve(nsph_grid* grid, int n);
+/* Hash d_points (n x 3 float32, particle order) with cell width `radius`
+ * (solver_xpbd.py:319 particle_grid.build(points, radius)).  The sort is stable:
+ * points of one cell keep ascending id order. Grows the reservation if n exceeds it. */
+int nsph_grid_build(nsph_grid* grid, const float* d_points, int n, float radius);
+/* d_ids[k] = id of the k-th point in cell-sorted order (hash_grid_point_id). */
+int nsph_grid_point_ids(const nsph_grid* grid, int* d_ids);
+int nsph_grid_num_points(const nsph_grid* grid, int* n);
+/* Neighbour lists of grid points: for each point i with (flags[i] & ACTIVE) (all points if
+ * d_flags is NULL), every j != i with ACTIVE flag and
+ *     dx*dx + dy*dy + dz*dz < radius*radius     (FP32, x_i - x_j, left-to-right, no FMA)
+ * in the candidate order of the grid query.  Lists are slot-major: d_idx[k*n + i],
+ * k < max_neighbors; d_counts[i] is the true count (may exceed max_neighbors, in which
+ * case the call returns NSPH_ERR_CAPACITY after filling the first max_neighbors).
+ * Mirrors the consumer loop of kernels_contact.py:68-73 / xpbd/kernels.py:257-264. */
+int nsph_grid_neighbors(const nsph_grid* grid, const int* d_flags, float radius,
+                        int max_neighbors, int* d_counts, int* d_idx);
+/* Same for arbitrary query points d_q (nq x 3): all grid points j with |q - x_j|^2 < r^2. */
+int nsph_grid_query(const nsph_grid* grid, const float* d_q, int nq, float radius,
+                    int max_neighbors, int* d_counts, int* d_idx);
+int nsph_grid_destroy(nsph_grid* grid);
+
+/* ---- SPH solver (behind SolverBase.step, solver.py:301-316) ------------------------ */
+
+enum nsph_warm_start { NSPH_WARM_ZERO = 0, NSPH_WARM_PREV = 1, NSPH_WARM_MLP = 2 };
+
+typedef struct nsph_model_desc {
+    int n_particles;          /* model.particle_count                                 */
+    int n_worlds;             /* model.world_count (>= 1); world -1 maps to world 0   */
+    const float* d_q;         /* model.particle_q   (N x 3) positions at solver creation */
+    const float* d_qd;        /* model.particle_qd  (N x 3)                           */
+    const float* d_mass;      /* model.particle_mass (N)                              */
+    const float* d_inv_mass;  /* model.particle_inv_mass (N)                          */
+    const int* d_flags;       /* model.particle_flags (N)                             */
+    const int* d_world;       /* model.particle_world (N), NULL = all world 0        */
+    const float* h_gravity;   /* host, n_worlds x 3 (model.gravity)                   */
+    float v_max;              /* model.particle_max_velocity                          */
+} nsph_model_desc;
+
+typedef struct nsph_sph_params {
+    const double* h_support;        /* host, n_worlds: kernel support radius H [m]        */
+    const double* h_boundary_volume;/* host, n_worlds: rest volume of a boundary particle */
+    const double* h_domain_lo;      /* host, n_worlds x 3: cell-table box lower corner    */
+    const double* h_domain_hi;      /* host, n_worlds x 3: cell-table box upper corner    */
+    double rest_density;            /* rho_0 [kg/m^3]                                     */
+    double viscosity;               /* kinematic viscosity nu [m^2/s]                     */
+    double omega;                   /* relaxed-Jacobi factor (IISPH: 0.5)                 */
+    double tolerance;               /* mean compression / rho_0 stop test (1e-3 = 0.1%)   */
+    int min_iters;                  /* IISPH: 2                                           */
+    int max_iters;
+    int max_neighbors;              /* fluid-fluid list capacity per particle             */
+    int warm_start;                 /* enum nsph_warm_start                               */
+} nsph_sph_params;
+
+typedef struct nsph_step_report {
+    int status;               /* NSPH_OK / NSPH_ERR_CAPACITY / NSPH_ERR_NOT_CONVERGED / NSPH_ERR_NONFINITE */
+    int max_iterations;       /* max over worlds of Jacobi updates applied            */
+    int sum_iterations;       /* sum over worlds                                      */
+    int unconverged_worlds;   /* worlds that stopped at max_iters                     */
+    int max_neighbor_count;   /* max fluid-fluid neighbours of any particle           */
+    int neighbor_overflow;    /* 1 if any list exceeded max_neighbors                 */
+    double max_density_error; /* max over worlds of the final mean compression / rho_0 */
+} nsph_step_report;
+
+/* Create the solver: classifies particles (fluid = ACTIVE and inv_mass > 0; static
+ * boundary = ACTIVE, inv_mass == 0 and zero velocity; the rest inactive) and hashes the
+ * static boundary once. Fails with NSPH_ERR_INVALID for a zero-mass ACTIVE particle with
+ * non-zero velocity (moving boundaries are not supported). */
+int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* model, const nsph_sph_params* params,
+                       nsph_solver** out);
+/* One step: state_in (d_q_in, d_qd_in) -> state_out (d_q_out, d_qd_out, d_f_out).
+ * d_f_out receives the SPH force (viscosity + pressure, N) of fluid particles and 0
+ * elsewhere; it may be NULL. Inactive particles: q_out = q_in, qd_out untouched
+ * (solver.py:27-29). check_every: read the convergence flag on the host every
+ * check_every Jacobi iterations (0 = launch max_iters early-exit iterations without any
+ * host read; the step is then capture-safe). report may be NULL (no host sync). */
+int nsph_solver_step(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
+                     float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
+                     int check_every, nsph_step_report* report);
+/* Collect the report of the last step (synchronises the stream). */
+int nsph_solver_report(nsph_solver* s, nsph_step_report* report);
+/* Per-particle fields of the last step in particle order (any pointer may be NULL):
+ * density rho, pressure p, fluid-fluid neighbour count. Non-fluid entries are 0. */
+int nsph_solver_fields(nsph_solver* s, float* d_rho, float* d_p, int* d_nbr_count);
+/* Per-world Jacobi iterations and final mean compression of the last step (host arrays). */
+int nsph_solver_world_stats(nsph_solver* s, int* h_iters, double* h_err, int* h_state);
+/* Fluid-fluid neighbour lists of the last step in particle ids, slot-major
+ * d_idx[k*N + id] for k < max_neighbors, d_counts[id] (0 for non-fluid). */
+int nsph_solver_neighbors(nsph_solver* s, int* d_counts, int* d_idx);
+int nsph_solver_set_warm_start(nsph_solver* s, int mode);
+/* Forget the previous-step pressure (warm start input). */
+int nsph_solver_reset_pressure(nsph_solver* s);
+int nsph_solver_num_fluid(const nsph_solver* s, int* n_fluid, int* n_boundary);
+/* Per-kernel-class GPU timing with HIP events recorded on the context stream around each
+ * launch group (off by default; enabling resets the totals). nsph_solver_timing
+ * synchronises and returns accumulated milliseconds and launch counts per class
+ * (nsph_kernel_count() classes named by nsph_kernel_name). No reference counterpart:
+ * the reference's timers are wall-clock ScopedTimer brackets (solver_xpbd.py:302,372). */
+int nsph_solver_set_timing(nsph_solver* s, int enable);
+int nsph_solver_timing(nsph_solver* s, double* h_ms, int* h_launches);
+int nsph_kernel_count(void);
+const char* nsph_kernel_name(int k);
+/* Jacobi iterations launched by the last step (>= the max over worlds of iterations
+ * applied when check_every > 1 or 0; the extra ones exit at once). */
+int nsph_solver_iterations_launched(const nsph_solver* s, int* n);
+int nsph_solver_destroy(nsph_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSPH_H */

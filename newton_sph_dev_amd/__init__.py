@@ -1,0 +1,23 @@
+"""newton_sph_dev_amd: MI355X-native SPH particle step behind Newton's Model/State/Solver API.
+
+Host code is Python; the step runs in libnsph.so (hand-written gfx950 HIP kernels, C ABI
+declared in include/nsph.h) on PyTorch-ROCm tensors.
+"""
+
+from .hashgrid import HashGrid
+from .sim import Axis, Contacts, Control, Model, ModelBuilder, ParticleFlags, State
+from .solvers import SolverBase, SolverNotifyFlags, SolverSPH
+
+__all__ = [
+    "Axis",
+    "Contacts",
+    "Control",
+    "HashGrid",
+    "Model",
+    "ModelBuilder",
+    "ParticleFlags",
+    "SolverBase",
+    "SolverNotifyFlags",
+    "SolverSPH",
+    "State",
+]

@@ -1,0 +1,1195 @@
+// SPH particle step for gfx950: IISPH (Ihmsen et al., IEEE TVCG 20(3), 2014) with a cubic
+// spline kernel (Monaghan 1992) of support H, static boundary particles (Akinci et al. 2012
+// style pseudo-mass psi = rho0 * V_b), relaxed Jacobi pressure iterations and a
+// deterministic FP64 mean-compression stop test.
+//
+// It sits behind SolverBase.step(state_in, state_out, control, contacts, dt)
+// (newton/_src/solvers/solver.py:301-316) and ends with the reference particle integrator
+// (solver.py:10-47, integrate_particles): v1 = v0 + (f*w + g*step(-w))*dt, |v1| clamp,
+// x1 = x0 + v1*dt, inactive particles keep x and leave v untouched.
+//
+// Layout (HBM, fluid particles in cell-sorted order s; one world = one contiguous run):
+//   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, 0)
+//   rho[s], aii[s], src[s], p[s], pr[s]=p/rho^2   float
+//   gb[s]   float4 (sum_b gradW_ib, boundary count)
+//   avisc[s], vadv[s], ap[s]  float4
+//   nbr[k*Nf + s] int  fluid-fluid neighbour list, slot-major (lanes of a wave read
+//                      consecutive addresses for the same slot)
+// Boundary particles are hashed once into their own table (bxs, btable); only the density
+// pass visits them, and it folds them into gb, so the other passes never touch them.
+//
+// Work partition: a "chunk" is 256 consecutive sorted fluid particles of one world.
+// Kernels run one workgroup per chunk, so the world of a workgroup is uniform and every
+// per-world quantity (cell box, kernel constants, convergence flag) is a scalar load.
+// The per-iteration FP64 reduction sums each chunk with a fixed LDS tree, then each world's
+// chunks with one wave in a fixed lane order, so the result is bitwise repeatable and equal
+// between a world run alone and the same world inside a batch.
+
+#include <math.h>
+#include <string.h>
+
+#include <cmath>
+
+#include <vector>
+
+#include "common.h"
+
+namespace nsph {
+
+constexpr int CHUNK = 256;
+
+struct PhysWorld {
+    float H, H2, invH, sigma;        // support, H^2, 1/H, 8/(pi H^3)
+    float sigma2, gsig2, gsig6, W0;  // 2 sigma, sigma/H^2, 6 sigma/H, W(0) = sigma
+    float psi, eps2, visc10, rq;     // rho0*V_b, 0.01 H^2, 10 nu, padded query radius
+    float g[3], rho0;
+};
+
+// W(r) and (dW/dr)/r of the cubic spline for r^2 = r2 < H^2.
+__device__ inline void kern_wg(const PhysWorld& P, float r2, float* W, float* gf) {
+    const float r = sqrtf(r2);
+    const float q = r * P.invH;
+    if (q <= 0.5f) {
+        float a = 6.0f * q;
+        a = a * q;
+        a = a * (q - 1.0f);
+        *W = P.sigma * (a + 1.0f);
+        *gf = P.gsig2 * (18.0f * q - 12.0f);
+    } else {
+        const float u = fmaxf(1.0f - q, 0.0f);
+        *W = P.sigma2 * ((u * u) * u);
+        *gf = -(P.gsig6 * (u * u)) / r;
+    }
+}
+
+__device__ inline float kern_gf(const PhysWorld& P, float r2) {
+    const float r = sqrtf(r2);
+    const float q = r * P.invH;
+    if (q <= 0.5f) return P.gsig2 * (18.0f * q - 12.0f);
+    const float u = fmaxf(1.0f - q, 0.0f);
+    return -(P.gsig6 * (u * u)) / r;
+}
+
+// Neumaier compensated add.
+__device__ inline void neumaier(float& s, float& c, float t) {
+    const float u = s + t;
+    if (fabsf(s) >= fabsf(t))
+        c += (s - u) + t;
+    else
+        c += (t - u) + s;
+    s = u;
+}
+
+__device__ inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+struct SphDev {
+    int Nf, cap;
+    // per-world tables
+    const GridWorld* gw;
+    const PhysWorld* phys;
+    const int* w_fstart;
+    const int* w_nf;
+    const int* w_cfirst;
+    const int* w_nchunks;
+    const int* chunk_world;
+    // boundary (static)
+    const int2* btable;
+    const float4* bxs;
+    // fluid grid
+    int2* ftable;
+    // sorted fluid arrays
+    float4* xs;
+    float4* vs;
+    float* rho;
+    float* aii;
+    float4* gb;
+    float4* avisc;
+    float4* vadv;
+    float* src;
+    float* p;
+    float* pr;
+    float4* ap;
+    float* pws;  // warm-start input in sorted order
+    int* nbr;
+    int* ncnt;
+    // iteration control
+    double* partial;
+    int* iters;
+    double* err;
+    int* conv;
+    int* act;
+    int* flags;  // [0] overflow, [1] max count, [2] non-finite
+};
+
+// Map workgroup -> (world, rank inside world, sorted index). Returns false past the world's end.
+__device__ inline bool chunk_slot(const SphDev& D, int* w, int* s) {
+    const int c = blockIdx.x;
+    const int ww = __builtin_amdgcn_readfirstlane(D.chunk_world[c]);
+    const int r = (c - D.w_cfirst[ww]) * CHUNK + (int)threadIdx.x;
+    *w = ww;
+    *s = D.w_fstart[ww] + r;
+    return r < D.w_nf[ww];
+}
+
+// ---- build ------------------------------------------------------------------------------
+
+__global__ void k_sph_keys(const float* __restrict__ q, const int* __restrict__ fid,
+                           const int* __restrict__ fworld, const GridWorld* __restrict__ gw,
+                           int n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int id = fid[k];
+    const GridWorld g = gw[fworld[k]];
+    keys[k] = cell_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+    vals[k] = (uint32_t)k;
+}
+
+__global__ void k_sph_reorder(const float* __restrict__ q, const float* __restrict__ qd,
+                              const float* __restrict__ mass, const int* __restrict__ fid,
+                              const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
+                              const uint32_t* __restrict__ skeys, int n, float4* __restrict__ xs,
+                              float4* __restrict__ vs, float* __restrict__ pws,
+                              int2* __restrict__ ftable) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = (int)perm[s];
+    const int id = fid[k];
+    xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
+    vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
+    pws[s] = pprev[k];
+    const uint32_t key = skeys[s];
+    if (s == 0 || skeys[s - 1] != key) ftable[key].x = s;
+    if (s == n - 1 || skeys[s + 1] != key) ftable[key].y = s + 1;
+}
+
+__global__ void k_sph_step_init(SphDev D, int W) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < W) {
+        D.iters[w] = 0;
+        D.err[w] = 0.0;
+        D.conv[w] = D.w_nf[w] == 0 ? 1 : 0;
+    }
+    if (w == 0) {
+        D.act[0] = 0;
+        D.act[1] = W;
+        D.flags[0] = 0;
+        D.flags[1] = 0;
+        D.flags[2] = 0;
+    }
+}
+
+// Fluid-fluid neighbour lists: j != i, |x_i - x_j|^2 < H^2 (FP32 predicate of grid_visit).
+__global__ __launch_bounds__(CHUNK) void k_sph_neighbors(SphDev D) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const GridWorld g = D.gw[w];
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    int c = 0;
+    const int Nf = D.Nf, cap = D.cap;
+    int* __restrict__ nbr = D.nbr;
+    grid_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
+               [&](int j, float, float, float, float) {
+                   if (j == s) return;
+                   if (c < cap) nbr[(size_t)c * Nf + s] = j;
+                   ++c;
+               });
+    D.ncnt[s] = c < cap ? c : cap;
+    if (c > cap) atomicOr(&D.flags[0], 1);
+    atomicMax(&D.flags[1], c);
+}
+
+// Density (compensated), the diagonal a_ii and the boundary gradient sum.
+//   rho_i = m_i W(0) + sum_j m_j W_ij + psi sum_b W_ib                (Neumaier, list order,
+//                                                                       then boundary order)
+//   Gf = sum_j m_j gradW_ij,  S = sum_j m_j |gradW_ij|^2,  Gb = sum_b gradW_ib
+//   G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
+// a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.
+__global__ __launch_bounds__(CHUNK) void k_sph_density(SphDev D, float dt2) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    const GridWorld g = D.gw[w];
+    const float4 xi = D.xs[s];
+    const int Nf = D.Nf;
+    float sum = xi.w * P.W0, cmp = 0.0f;
+    float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
+    const int n = D.ncnt[s];
+    for (int k = 0; k < n; ++k) {
+        const int j = D.nbr[(size_t)k * Nf + s];
+        const float4 xj = D.xs[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        float W, gf;
+        kern_wg(P, r2, &W, &gf);
+        neumaier(sum, cmp, xj.w * W);
+        const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+        gfx += xj.w * gx;
+        gfy += xj.w * gy;
+        gfz += xj.w * gz;
+        S += xj.w * dot3(gx, gy, gz, gx, gy, gz);
+    }
+    float gbx = 0.f, gby = 0.f, gbz = 0.f;
+    int nb = 0;
+    grid_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+               [&](int, float dx, float dy, float dz, float r2) {
+                   float W, gf;
+                   kern_wg(P, r2, &W, &gf);
+                   neumaier(sum, cmp, P.psi * W);
+                   gbx += gf * dx;
+                   gby += gf * dy;
+                   gbz += gf * dz;
+                   ++nb;
+               });
+    const float rho = sum + cmp;
+    const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
+    const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + xi.w * S;
+    D.rho[s] = rho;
+    D.aii[s] = -(dt2 * (num / (rho * rho)));
+    D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
+}
+
+// Viscosity and advected velocity:
+//   a_visc = 10 nu sum_j (m_j/rho_j) ((v_ij . x_ij) / (r^2 + 0.01 H^2)) gradW_ij
+//   v_adv  = v + dt (g + a_visc)
+__global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    const float4 vi = D.vs[s];
+    const int Nf = D.Nf;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const int n = D.ncnt[s];
+    for (int k = 0; k < n; ++k) {
+        const int j = D.nbr[(size_t)k * Nf + s];
+        const float4 xj = D.xs[j];
+        const float4 vj = D.vs[j];
+        const float rj = D.rho[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const float gf = kern_gf(P, r2);
+        const float vx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, dx, dy, dz);
+        const float c = (xj.w / rj) * (vx / (r2 + P.eps2));
+        ax += c * (gf * dx);
+        ay += c * (gf * dy);
+        az += c * (gf * dz);
+    }
+    const float avx = P.visc10 * ax, avy = P.visc10 * ay, avz = P.visc10 * az;
+    D.avisc[s] = make_float4(avx, avy, avz, 0.f);
+    D.vadv[s] = make_float4(vi.x + dt * (P.g[0] + avx), vi.y + dt * (P.g[1] + avy),
+                            vi.z + dt * (P.g[2] + avz), 0.f);
+}
+
+// Source term and warm start:
+//   s_i = (rho0 - rho_i) - dt (sum_j m_j (v_adv_i - v_adv_j) . gradW_ij + psi v_adv_i . Gb_i)
+//   p_i = 0 | 0.5 p_prev | external;  pr_i = p_i / rho_i^2
+__global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int warm) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    const float4 vi = D.vadv[s];
+    const int Nf = D.Nf;
+    float sum = 0.f;
+    const int n = D.ncnt[s];
+    for (int k = 0; k < n; ++k) {
+        const int j = D.nbr[(size_t)k * Nf + s];
+        const float4 xj = D.xs[j];
+        const float4 vj = D.vadv[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const float gf = kern_gf(P, r2);
+        sum += xj.w * dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, gf * dx, gf * dy, gf * dz);
+    }
+    const float4 gb = D.gb[s];
+    sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
+    const float rho = D.rho[s];
+    D.src[s] = (P.rho0 - rho) - dt * sum;
+    float p0 = 0.f;
+    if (warm == NSPH_WARM_PREV) p0 = 0.5f * D.pws[s];
+    else if (warm == NSPH_WARM_MLP) p0 = D.pws[s];
+    D.p[s] = p0;
+    D.pr[s] = p0 / (rho * rho);
+}
+
+// Pressure acceleration  a^p_i = -( sum_j m_j (pr_i + pr_j) gradW_ij + (psi pr_i) Gb_i ).
+__device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int s, float4 xi,
+                                 float* ox, float* oy, float* oz) {
+    const int Nf = D.Nf;
+    const float pri = D.pr[s];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const int n = D.ncnt[s];
+    for (int k = 0; k < n; ++k) {
+        const int j = D.nbr[(size_t)k * Nf + s];
+        const float4 xj = D.xs[j];
+        const float prj = D.pr[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const float gf = kern_gf(P, r2);
+        const float c = xj.w * (pri + prj);
+        ax += c * (gf * dx);
+        ay += c * (gf * dy);
+        az += c * (gf * dz);
+    }
+    const float4 gb = D.gb[s];
+    const float cb = P.psi * pri;
+    *ox = -(ax + cb * gb.x);
+    *oy = -(ay + cb * gb.y);
+    *oz = -(az + cb * gb.z);
+}
+
+__global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
+    int w, s;
+    const bool ok = chunk_slot(D, &w, &s);
+    if (D.conv[w]) return;
+    if (!ok) return;
+    const PhysWorld P = D.phys[w];
+    float ax, ay, az;
+    press_acc(D, P, s, D.xs[s], &ax, &ay, &az);
+    D.ap[s] = make_float4(ax, ay, az, 0.f);
+}
+
+// (A p)_i = dt^2 ( sum_j m_j (a^p_i - a^p_j) . gradW_ij + psi a^p_i . Gb_i )
+// e_i     = max((A p)_i - s_i, 0)   (compression of the predicted density, kg/m^3)
+// p_i    <- a_ii != 0 ? max(p_i + omega (s_i - (A p)_i) / a_ii, 0) : 0
+// partial[chunk] = sum of e_i over the chunk (FP64, fixed LDS tree).
+__global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, float omega) {
+    __shared__ double red[CHUNK];
+    int w, s;
+    const bool ok = chunk_slot(D, &w, &s);
+    if (D.conv[w]) return;
+    const int t = threadIdx.x;
+    double e = 0.0;
+    if (ok) {
+        const PhysWorld P = D.phys[w];
+        const float4 xi = D.xs[s];
+        const float4 ai = D.ap[s];
+        const int Nf = D.Nf;
+        float sum = 0.f;
+        const int n = D.ncnt[s];
+        for (int k = 0; k < n; ++k) {
+            const int j = D.nbr[(size_t)k * Nf + s];
+            const float4 xj = D.xs[j];
+            const float4 aj = D.ap[j];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            sum += xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, gf * dx, gf * dy, gf * dz);
+        }
+        const float4 gb = D.gb[s];
+        sum += P.psi * dot3(ai.x, ai.y, ai.z, gb.x, gb.y, gb.z);
+        const float Ap = dt2 * sum;
+        const float si = D.src[s];
+        e = (double)fmaxf(Ap - si, 0.0f);
+        const float a = D.aii[s];
+        float pn = 0.f;
+        if (a != 0.0f) pn = fmaxf(D.p[s] + omega * (si - Ap) / a, 0.0f);
+        D.p[s] = pn;
+        const float rho = D.rho[s];
+        D.pr[s] = pn / (rho * rho);
+    }
+    red[t] = e;
+    __syncthreads();
+    for (int off = CHUNK / 2; off > 0; off >>= 1) {
+        if (t < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+    if (t == 0) D.partial[blockIdx.x] = red[0];
+}
+
+// One wave per world: err_w = (sum of its chunk partials) / (n_w rho0); count the update;
+// decide convergence. act[it & 1] counts the worlds still iterating after iteration `it`.
+__global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters, int max_iters) {
+    const int lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.act[(it + 1) & 1] = 0;
+    if (w >= W) return;
+    if (D.conv[w]) return;
+    const int first = D.w_cfirst[w], nch = D.w_nchunks[w];
+    double acc = 0.0;
+    for (int k = lane; k < nch; k += 64) acc += D.partial[first + k];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) {
+        const double err = acc / ((double)D.w_nf[w] * (double)D.phys[w].rho0);
+        const int n_it = D.iters[w] + 1;
+        D.iters[w] = n_it;
+        D.err[w] = err;
+        int c = 0;
+        if (n_it >= min_iters && err <= tol) c = 1;
+        else if (n_it >= max_iters) c = 2;
+        D.conv[w] = c;
+        if (c == 0) atomicAdd(&D.act[it & 1], 1);
+    }
+}
+
+// Pressure acceleration from the final p, SPH force and the reference integrator
+// (solver.py:24-46):  f = m (a_visc + a^p);  v1 = v0 + (f w + g) dt;  |v1| <= v_max;
+// x1 = x0 + v1 dt.  Writes particle order.
+__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* __restrict__ perm,
+                                                     const int* __restrict__ fid,
+                                                     const float* __restrict__ inv_mass, float dt,
+                                                     float v_max, float* __restrict__ q_out,
+                                                     float* __restrict__ qd_out,
+                                                     float* __restrict__ f_out,
+                                                     float* __restrict__ pprev) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    float ax, ay, az;
+    press_acc(D, P, s, xi, &ax, &ay, &az);
+    const float4 av = D.avisc[s];
+    const float m = xi.w;
+    const float fx = m * (av.x + ax), fy = m * (av.y + ay), fz = m * (av.z + az);
+    const int k = (int)perm[s];
+    const int id = fid[k];
+    const float wi = inv_mass[id];
+    const float4 v0 = D.vs[s];
+    float v1x = v0.x + (fx * wi + P.g[0]) * dt;
+    float v1y = v0.y + (fy * wi + P.g[1]) * dt;
+    float v1z = v0.z + (fz * wi + P.g[2]) * dt;
+    const float mag = sqrtf(dot3(v1x, v1y, v1z, v1x, v1y, v1z));
+    if (mag > v_max) {
+        const float sc = v_max / mag;
+        v1x *= sc;
+        v1y *= sc;
+        v1z *= sc;
+    }
+    const float x1 = xi.x + v1x * dt, y1 = xi.y + v1y * dt, z1 = xi.z + v1z * dt;
+    q_out[3 * id] = x1;
+    q_out[3 * id + 1] = y1;
+    q_out[3 * id + 2] = z1;
+    qd_out[3 * id] = v1x;
+    qd_out[3 * id + 1] = v1y;
+    qd_out[3 * id + 2] = v1z;
+    if (f_out) {
+        f_out[3 * id] = fx;
+        f_out[3 * id + 1] = fy;
+        f_out[3 * id + 2] = fz;
+    }
+    pprev[k] = D.p[s];
+    if (!(isfinite(x1) && isfinite(y1) && isfinite(z1) && isfinite(v1x) && isfinite(v1y) &&
+          isfinite(v1z)))
+        atomicOr(&D.flags[2], 1);
+}
+
+// Non-fluid particles, reference integrator with zero force (solver.py:24-46).
+__global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const float* __restrict__ q,
+                                  const float* __restrict__ qd, const float* __restrict__ inv_mass,
+                                  const int* __restrict__ flags, const int* __restrict__ world,
+                                  const PhysWorld* __restrict__ phys, int W, float dt, float v_max,
+                                  float* __restrict__ q_out, float* __restrict__ qd_out,
+                                  float* __restrict__ f_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int id = ids[k];
+    const float x0 = q[3 * id], y0 = q[3 * id + 1], z0 = q[3 * id + 2];
+    if (f_out) {
+        f_out[3 * id] = 0.f;
+        f_out[3 * id + 1] = 0.f;
+        f_out[3 * id + 2] = 0.f;
+    }
+    if ((flags[id] & NSPH_PARTICLE_ACTIVE) == 0) {
+        q_out[3 * id] = x0;
+        q_out[3 * id + 1] = y0;
+        q_out[3 * id + 2] = z0;
+        return;
+    }
+    int wd = world ? world[id] : 0;
+    wd = wd < 0 ? 0 : (wd >= W ? W - 1 : wd);
+    const PhysWorld& P = phys[wd];
+    const float wi = inv_mass[id];
+    const float stp = (-wi < 0.0f) ? 1.0f : 0.0f;
+    float v1x = qd[3 * id] + (0.0f * wi + P.g[0] * stp) * dt;
+    float v1y = qd[3 * id + 1] + (0.0f * wi + P.g[1] * stp) * dt;
+    float v1z = qd[3 * id + 2] + (0.0f * wi + P.g[2] * stp) * dt;
+    const float mag = sqrtf(dot3(v1x, v1y, v1z, v1x, v1y, v1z));
+    if (mag > v_max) {
+        const float sc = v_max / mag;
+        v1x *= sc;
+        v1y *= sc;
+        v1z *= sc;
+    }
+    q_out[3 * id] = x0 + v1x * dt;
+    q_out[3 * id + 1] = y0 + v1y * dt;
+    q_out[3 * id + 2] = z0 + v1z * dt;
+    qd_out[3 * id] = v1x;
+    qd_out[3 * id + 1] = v1y;
+    qd_out[3 * id + 2] = v1z;
+}
+
+// Boundary build (once): keys, then reorder into bxs and mark btable.
+__global__ void k_bnd_keys(const float* __restrict__ q, const int* __restrict__ bid,
+                           const int* __restrict__ bworld, const GridWorld* __restrict__ gw, int n,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int id = bid[k];
+    const GridWorld g = gw[bworld[k]];
+    keys[k] = cell_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+    vals[k] = (uint32_t)k;
+}
+
+__global__ void k_bnd_reorder(const float* __restrict__ q, const int* __restrict__ bid,
+                              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
+                              int n, float4* __restrict__ bxs, int2* __restrict__ btable) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = bid[perm[s]];
+    bxs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], 0.f);
+    const uint32_t key = skeys[s];
+    if (s == 0 || skeys[s - 1] != key) btable[key].x = s;
+    if (s == n - 1 || skeys[s + 1] != key) btable[key].y = s + 1;
+}
+
+// Sorted -> particle order for the debugging / dump entry points.
+__global__ void k_sph_fields(const uint32_t* __restrict__ perm, const int* __restrict__ fid, int n,
+                             const float* __restrict__ rho, const float* __restrict__ p,
+                             const int* __restrict__ ncnt, float* __restrict__ o_rho,
+                             float* __restrict__ o_p, int* __restrict__ o_cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = fid[perm[s]];
+    if (o_rho) o_rho[id] = rho[s];
+    if (o_p) o_p[id] = p[s];
+    if (o_cnt) o_cnt[id] = ncnt[s];
+}
+
+__global__ void k_sph_export_nbrs(const uint32_t* __restrict__ perm, const int* __restrict__ fid,
+                                  int Nf, int N, int cap, const int* __restrict__ nbr,
+                                  const int* __restrict__ ncnt, int* __restrict__ counts,
+                                  int* __restrict__ idx) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Nf) return;
+    const int id = fid[perm[s]];
+    const int n = ncnt[s];
+    counts[id] = n;
+    for (int k = 0; k < n && k < cap; ++k) idx[(size_t)k * N + id] = fid[perm[nbr[(size_t)k * Nf + s]]];
+}
+
+__global__ void k_f32_zero(float* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+static PhysWorld make_phys(double H, double Vb, double rho0, double nu, const float* g) {
+    PhysWorld P{};
+    const double sigma = 8.0 / (M_PI * H * H * H);
+    P.H = (float)H;
+    P.H2 = (float)(H * H);
+    P.invH = (float)(1.0 / H);
+    P.sigma = (float)sigma;
+    P.sigma2 = (float)(2.0 * sigma);
+    P.gsig2 = (float)(sigma / (H * H));
+    P.gsig6 = (float)(6.0 * sigma / H);
+    P.W0 = (float)sigma;
+    P.psi = (float)(rho0 * Vb);
+    P.eps2 = (float)(0.01 * H * H);
+    P.visc10 = (float)(10.0 * nu);
+    P.rq = (float)H;
+    P.g[0] = g[0];
+    P.g[1] = g[1];
+    P.g[2] = g[2];
+    P.rho0 = (float)rho0;
+    return P;
+}
+
+}  // namespace nsph
+
+using namespace nsph;
+
+// Kernel classes timed by nsph_solver_set_timing (names in nsph_kernel_name).
+enum KernelId {
+    KT_BUILD = 0,  // cell clear + keys + radix sort + reorder/mark
+    KT_NEIGHBORS,
+    KT_DENSITY,
+    KT_ADVECT,
+    KT_SOURCE,
+    KT_PACC,
+    KT_PUPDATE,
+    KT_REDUCE,
+    KT_FINAL,
+    KT_PASSTHROUGH,
+    KT_COUNT
+};
+static const char* kKernelNames[KT_COUNT] = {"build", "neighbors", "density", "advect", "source",
+                                            "pressure_acc", "pressure_update", "reduce", "final",
+                                            "passthrough"};
+
+struct EventPair {
+    int kind;
+    hipEvent_t a, b;
+};
+
+struct nsph_solver {
+    nsph_ctx* ctx = nullptr;
+    // per-kernel timing (off unless enabled)
+    int timing = 0;
+    std::vector<EventPair> ev_pool;
+    size_t ev_used = 0;
+    double t_ms[KT_COUNT] = {0};
+    long t_count[KT_COUNT] = {0};
+    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0;
+    int key_bits = 0;
+    size_t total_cells = 0;
+    double rho0 = 1000, omega = 0.5, tol = 1e-3;
+    int min_iters = 2, max_iters = 100, warm = 0;
+    float v_max = 1e5f;
+    const float* d_mass = nullptr;  // model arrays (caller-owned, must outlive the solver)
+    const float* d_inv_mass = nullptr;
+    const int* d_flags = nullptr;
+    const int* d_world = nullptr;
+    std::vector<int> h_nf, h_iters_last;
+    // device tables
+    DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
+    DevBuf fid, fworld, oid;
+    DevBuf btable, bxs, ftable;
+    DevBuf keys, vals;
+    uint32_t* skeys = nullptr;
+    uint32_t* perm = nullptr;
+    int have_prev_keys = 0;
+    RadixSort rs;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, pr, ap, pws, pprev, nbr, ncnt;
+    DevBuf partial, iters, err, conv, act, flags;
+    int* h_pinned = nullptr;  // [0] act value
+    int stepped = 0;
+    int last_iters_launched = 0;
+};
+
+static SphDev make_dev(nsph_solver* S) {
+    SphDev D{};
+    D.Nf = S->Nf;
+    D.cap = S->cap;
+    D.gw = bp<GridWorld>(S->gw);
+    D.phys = bp<PhysWorld>(S->phys);
+    D.w_fstart = bp<int>(S->w_fstart);
+    D.w_nf = bp<int>(S->w_nf);
+    D.w_cfirst = bp<int>(S->w_cfirst);
+    D.w_nchunks = bp<int>(S->w_nchunks);
+    D.chunk_world = bp<int>(S->chunk_world);
+    D.btable = bp<int2>(S->btable);
+    D.bxs = bp<float4>(S->bxs);
+    D.ftable = bp<int2>(S->ftable);
+    D.xs = bp<float4>(S->xs);
+    D.vs = bp<float4>(S->vs);
+    D.rho = bp<float>(S->rho);
+    D.aii = bp<float>(S->aii);
+    D.gb = bp<float4>(S->gb);
+    D.avisc = bp<float4>(S->avisc);
+    D.vadv = bp<float4>(S->vadv);
+    D.src = bp<float>(S->src);
+    D.p = bp<float>(S->p);
+    D.pr = bp<float>(S->pr);
+    D.ap = bp<float4>(S->ap);
+    D.pws = bp<float>(S->pws);
+    D.nbr = bp<int>(S->nbr);
+    D.ncnt = bp<int>(S->ncnt);
+    D.partial = bp<double>(S->partial);
+    D.iters = bp<int>(S->iters);
+    D.err = bp<double>(S->err);
+    D.conv = bp<int>(S->conv);
+    D.act = bp<int>(S->act);
+    D.flags = bp<int>(S->flags);
+    return D;
+}
+
+template <class T>
+static int upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    NSPH_TRY(buf_ensure(b, v.size() * sizeof(T) + 16));
+    if (!v.empty())
+        NSPH_HIP(hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return NSPH_OK;
+}
+
+static void solver_free(nsph_solver* S) {
+    DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
+                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs,
+                      &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
+                      &S->avisc, &S->vadv, &S->src, &S->p, &S->pr, &S->ap, &S->pws, &S->pprev,
+                      &S->nbr, &S->ncnt, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
+                      &S->flags};
+    for (DevBuf* b : bufs) buf_free(*b);
+    radix_free(S->rs);
+    if (S->h_pinned) (void)hipHostFree(S->h_pinned);
+    S->h_pinned = nullptr;
+    for (EventPair& e : S->ev_pool) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    S->ev_pool.clear();
+}
+
+// Timing brackets: record an event pair around a group of launches (timing mode only).
+static int tic(nsph_solver* S, int kind, EventPair** out) {
+    *out = nullptr;
+    if (!S->timing) return NSPH_OK;
+    if (S->ev_used == S->ev_pool.size()) {
+        EventPair e{};
+        NSPH_HIP(hipEventCreate(&e.a));
+        NSPH_HIP(hipEventCreate(&e.b));
+        S->ev_pool.push_back(e);
+    }
+    EventPair* e = &S->ev_pool[S->ev_used++];
+    e->kind = kind;
+    NSPH_HIP(hipEventRecord(e->a, S->ctx->stream));
+    *out = e;
+    return NSPH_OK;
+}
+
+static int toc(nsph_solver* S, EventPair* e) {
+    if (e) NSPH_HIP(hipEventRecord(e->b, S->ctx->stream));
+    return NSPH_OK;
+}
+
+static int flush_timing(nsph_solver* S) {
+    if (S->ev_used == 0) return NSPH_OK;
+    NSPH_HIP(hipStreamSynchronize(S->ctx->stream));
+    for (size_t i = 0; i < S->ev_used; ++i) {
+        float ms = 0.f;
+        NSPH_HIP(hipEventElapsedTime(&ms, S->ev_pool[i].a, S->ev_pool[i].b));
+        S->t_ms[S->ev_pool[i].kind] += ms;
+        S->t_count[S->ev_pool[i].kind] += 1;
+    }
+    S->ev_used = 0;
+    return NSPH_OK;
+}
+
+#define TIMED(kind, ...)                                                                 \
+    do {                                                                                 \
+        EventPair* ep_;                                                                  \
+        NSPH_TRY(tic(S, (kind), &ep_));                                                  \
+        __VA_ARGS__;                                                                     \
+        NSPH_TRY(toc(S, ep_));                                                           \
+    } while (0)
+
+static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_params* prm) {
+    hipStream_t st = S->ctx->stream;
+    const int N = m->n_particles, W = m->n_worlds;
+    S->N = N;
+    S->W = W;
+    S->rho0 = prm->rest_density;
+    S->omega = prm->omega;
+    S->tol = prm->tolerance;
+    S->min_iters = prm->min_iters;
+    S->max_iters = prm->max_iters;
+    S->warm = prm->warm_start;
+    S->cap = prm->max_neighbors;
+    S->v_max = m->v_max;
+    S->d_mass = m->d_mass;
+    S->d_inv_mass = m->d_inv_mass;
+    S->d_flags = m->d_flags;
+    S->d_world = m->d_world;
+
+    // classify on the host (creation time only)
+    std::vector<float> q(3 * (size_t)N), qd(3 * (size_t)N), inv_mass(N);
+    std::vector<int> flags(N), world(N, 0);
+    if (N > 0) {
+        NSPH_HIP(hipMemcpyAsync(q.data(), m->d_q, q.size() * 4, hipMemcpyDeviceToHost, st));
+        NSPH_HIP(hipMemcpyAsync(qd.data(), m->d_qd, qd.size() * 4, hipMemcpyDeviceToHost, st));
+        NSPH_HIP(hipMemcpyAsync(inv_mass.data(), m->d_inv_mass, (size_t)N * 4, hipMemcpyDeviceToHost, st));
+        NSPH_HIP(hipMemcpyAsync(flags.data(), m->d_flags, (size_t)N * 4, hipMemcpyDeviceToHost, st));
+        if (m->d_world)
+            NSPH_HIP(hipMemcpyAsync(world.data(), m->d_world, (size_t)N * 4, hipMemcpyDeviceToHost, st));
+        NSPH_HIP(hipStreamSynchronize(st));
+    }
+    std::vector<int> fid, fworld, bid, bworld, oid;
+    std::vector<int> nf(W, 0), nb(W, 0);
+    for (int i = 0; i < N; ++i) {
+        int w = world[i] < 0 ? 0 : world[i];
+        NSPH_CHECK(w < W, NSPH_ERR_INVALID, "nsph_solver_create: particle %d has world %d >= n_worlds %d",
+                   i, world[i], W);
+        const bool active = (flags[i] & NSPH_PARTICLE_ACTIVE) != 0;
+        if (active && inv_mass[i] > 0.f) {
+            fid.push_back(i);
+            fworld.push_back(w);
+            nf[w]++;
+        } else {
+            oid.push_back(i);
+            if (active) {
+                NSPH_CHECK(qd[3 * (size_t)i] == 0.f && qd[3 * (size_t)i + 1] == 0.f && qd[3 * (size_t)i + 2] == 0.f,
+                           NSPH_ERR_INVALID,
+                           "nsph_solver_create: zero-mass particle %d has non-zero velocity; "
+                           "moving boundary particles are not supported", i);
+                bid.push_back(i);
+                bworld.push_back(w);
+                nb[w]++;
+            }
+        }
+    }
+    // fluid of each world must be contiguous in sorted order: keys are world-major, so sorting
+    // by key groups worlds.  Fluid ids are ascending; the stable sort keeps id order per cell.
+    S->Nf = (int)fid.size();
+    S->Nb = (int)bid.size();
+    S->No = (int)oid.size();
+    S->h_nf = nf;
+
+    // per-world grid boxes and kernel constants
+    std::vector<GridWorld> gws(W);
+    std::vector<PhysWorld> phys(W);
+    size_t off = 0;
+    for (int w = 0; w < W; ++w) {
+        const double H = prm->h_support[w];
+        NSPH_CHECK(H > 0.0 && std::isfinite(H), NSPH_ERR_INVALID, "nsph_solver_create: bad support radius in world %d", w);
+        const double cw = H * (1.0 + 1.0 / 128.0);
+        GridWorld g{};
+        for (int a = 0; a < 3; ++a) {
+            const double lo = prm->h_domain_lo[3 * w + a], hi = prm->h_domain_hi[3 * w + a];
+            NSPH_CHECK(hi > lo, NSPH_ERR_INVALID, "nsph_solver_create: empty domain in world %d", w);
+            double d = std::ceil((hi - lo) / cw) + 1.0;
+            if (d < 3.0) d = 3.0;
+            NSPH_CHECK(d < 1.0e6, NSPH_ERR_INVALID, "nsph_solver_create: domain too large for the cell size in world %d", w);
+            g.lo[a] = (float)lo;
+            g.dim[a] = (int)d;
+        }
+        g.inv_w = (float)(1.0 / cw);
+        g.wrap = 0;
+        g.cell_off = (int)off;
+        off += (size_t)g.dim[0] * g.dim[1] * g.dim[2];
+        NSPH_CHECK(off < (size_t)1 << 31, NSPH_ERR_INVALID, "nsph_solver_create: more than 2^31 cells in total");
+        gws[w] = g;
+        phys[w] = make_phys(H, prm->h_boundary_volume[w], prm->rest_density, prm->viscosity,
+                            m->h_gravity + 3 * w);
+    }
+    S->total_cells = off;
+    S->key_bits = bits_for((uint32_t)(off - 1));
+
+    // chunk map
+    std::vector<int> fstart(W), cfirst(W), nch(W), chunk_world;
+    int acc = 0, cacc = 0;
+    for (int w = 0; w < W; ++w) {
+        fstart[w] = acc;
+        cfirst[w] = cacc;
+        nch[w] = (nf[w] + CHUNK - 1) / CHUNK;
+        for (int c = 0; c < nch[w]; ++c) chunk_world.push_back(w);
+        acc += nf[w];
+        cacc += nch[w];
+    }
+    S->C = cacc;
+
+    NSPH_TRY(upload(S->gw, gws, st));
+    NSPH_TRY(upload(S->phys, phys, st));
+    NSPH_TRY(upload(S->w_fstart, fstart, st));
+    NSPH_TRY(upload(S->w_nf, nf, st));
+    NSPH_TRY(upload(S->w_cfirst, cfirst, st));
+    NSPH_TRY(upload(S->w_nchunks, nch, st));
+    NSPH_TRY(upload(S->chunk_world, chunk_world, st));
+    NSPH_TRY(upload(S->fid, fid, st));
+    NSPH_TRY(upload(S->fworld, fworld, st));
+    NSPH_TRY(upload(S->oid, oid, st));
+
+    const int Nf = S->Nf, Nb = S->Nb;
+    const int nmax = Nf > Nb ? Nf : Nb;
+    NSPH_TRY(radix_reserve(S->rs, nmax > 0 ? nmax : 1));
+    NSPH_TRY(buf_ensure(S->keys, (size_t)(nmax + 1) * 4));
+    NSPH_TRY(buf_ensure(S->vals, (size_t)(nmax + 1) * 4));
+    NSPH_TRY(buf_ensure(S->btable, (off + 1) * sizeof(int2)));
+    NSPH_TRY(buf_ensure(S->ftable, (off + 1) * sizeof(int2)));
+    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->btable), off + 1, make_int2(0, 0));
+    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ftable), off + 1, make_int2(0, 0));
+    NSPH_TRY(buf_ensure(S->bxs, (size_t)(Nb + 1) * 16));
+
+    // static boundary: hash once
+    if (Nb > 0) {
+        DevBuf dbid, dbworld;
+        int rc = upload(dbid, bid, st);
+        if (rc == NSPH_OK) rc = upload(dbworld, bworld, st);
+        if (rc == NSPH_OK) {
+            hipLaunchKernelGGL(k_bnd_keys, dim3((Nb + 255) / 256), dim3(256), 0, st, m->d_q,
+                               bp<int>(dbid), bp<int>(dbworld), bp<GridWorld>(S->gw), Nb,
+                               bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+            uint32_t *sk, *sp;
+            rc = radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nb, S->key_bits, st, &sk, &sp);
+            if (rc == NSPH_OK) {
+                hipLaunchKernelGGL(k_bnd_reorder, dim3((Nb + 255) / 256), dim3(256), 0, st, m->d_q,
+                                   bp<int>(dbid), sp, sk, Nb, bp<float4>(S->bxs), bp<int2>(S->btable));
+                if (hipStreamSynchronize(st) != hipSuccess) rc = NSPH_ERR_HIP;
+            }
+        }
+        buf_free(dbid);
+        buf_free(dbworld);
+        NSPH_TRY(rc);
+        NSPH_HIP(hipGetLastError());
+    }
+
+    const size_t nf1 = (size_t)Nf + 1;
+    NSPH_TRY(buf_ensure(S->xs, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->vs, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->rho, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->aii, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->gb, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->avisc, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->vadv, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->src, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->p, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->pr, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->nbr, nf1 * (size_t)S->cap * 4));
+    NSPH_TRY(buf_ensure(S->ncnt, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->partial, ((size_t)S->C + 1) * 8));
+    NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
+    NSPH_TRY(buf_ensure(S->err, ((size_t)W + 1) * 8));
+    NSPH_TRY(buf_ensure(S->conv, ((size_t)W + 1) * 4));
+    NSPH_TRY(buf_ensure(S->act, 16));
+    NSPH_TRY(buf_ensure(S->flags, 16));
+    hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((nf1 + 255) / 256)), dim3(256), 0, st, bp<float>(S->pprev), nf1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
+    NSPH_HIP(hipGetLastError());
+    NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 64, hipHostMallocDefault));
+    NSPH_HIP(hipStreamSynchronize(st));
+    return NSPH_OK;
+}
+
+static int collect_report(nsph_solver* S, nsph_step_report* rep) {
+    hipStream_t st = S->ctx->stream;
+    const int W = S->W;
+    std::vector<int> iters(W), conv(W);
+    std::vector<double> err(W);
+    int fl[4] = {0, 0, 0, 0};
+    NSPH_HIP(hipMemcpyAsync(iters.data(), S->iters.ptr, (size_t)W * 4, hipMemcpyDeviceToHost, st));
+    NSPH_HIP(hipMemcpyAsync(conv.data(), S->conv.ptr, (size_t)W * 4, hipMemcpyDeviceToHost, st));
+    NSPH_HIP(hipMemcpyAsync(err.data(), S->err.ptr, (size_t)W * 8, hipMemcpyDeviceToHost, st));
+    NSPH_HIP(hipMemcpyAsync(fl, S->flags.ptr, 12, hipMemcpyDeviceToHost, st));
+    NSPH_HIP(hipStreamSynchronize(st));
+    nsph_step_report r{};
+    for (int w = 0; w < W; ++w) {
+        if (S->h_nf[w] == 0) continue;
+        r.max_iterations = iters[w] > r.max_iterations ? iters[w] : r.max_iterations;
+        r.sum_iterations += iters[w];
+        if (conv[w] != 1) r.unconverged_worlds++;
+        if (err[w] > r.max_density_error) r.max_density_error = err[w];
+    }
+    r.neighbor_overflow = fl[0];
+    r.max_neighbor_count = fl[1];
+    r.status = NSPH_OK;
+    if (r.unconverged_worlds) r.status = NSPH_ERR_NOT_CONVERGED;
+    if (fl[0]) r.status = NSPH_ERR_CAPACITY;
+    if (fl[2]) r.status = NSPH_ERR_NONFINITE;
+    *rep = r;
+    return NSPH_OK;
+}
+
+extern "C" {
+
+int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* m, const nsph_sph_params* prm,
+                       nsph_solver** out) {
+    NSPH_CHECK(ctx && m && prm && out, NSPH_ERR_INVALID, "nsph_solver_create: NULL argument");
+    NSPH_CHECK(m->n_particles >= 0 && m->n_worlds >= 1, NSPH_ERR_INVALID,
+               "nsph_solver_create: need n_particles >= 0 and n_worlds >= 1");
+    NSPH_CHECK(m->n_particles == 0 || (m->d_q && m->d_qd && m->d_mass && m->d_inv_mass && m->d_flags),
+               NSPH_ERR_INVALID, "nsph_solver_create: missing model array");
+    NSPH_CHECK(m->h_gravity && prm->h_support && prm->h_boundary_volume && prm->h_domain_lo &&
+                   prm->h_domain_hi, NSPH_ERR_INVALID, "nsph_solver_create: missing per-world parameter");
+    NSPH_CHECK(prm->max_neighbors >= 1 && prm->max_neighbors <= 4096, NSPH_ERR_INVALID,
+               "nsph_solver_create: max_neighbors must be in [1, 4096]");
+    NSPH_CHECK(prm->max_iters >= 1 && prm->min_iters >= 0 && prm->min_iters <= prm->max_iters,
+               NSPH_ERR_INVALID, "nsph_solver_create: need 0 <= min_iters <= max_iters, max_iters >= 1");
+    NSPH_CHECK(prm->rest_density > 0 && prm->omega > 0 && prm->tolerance >= 0, NSPH_ERR_INVALID,
+               "nsph_solver_create: rest_density, omega must be > 0 and tolerance >= 0");
+    NSPH_CHECK(prm->warm_start >= NSPH_WARM_ZERO && prm->warm_start <= NSPH_WARM_MLP, NSPH_ERR_INVALID,
+               "nsph_solver_create: unknown warm_start mode %d", prm->warm_start);
+    NSPH_HIP(hipSetDevice(ctx->device));
+    nsph_solver* S = new nsph_solver();
+    S->ctx = ctx;
+    int rc = solver_init(S, m, prm);
+    if (rc != NSPH_OK) {
+        solver_free(S);
+        delete S;
+        return rc;
+    }
+    *out = S;
+    return NSPH_OK;
+}
+
+int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
+                     float* qd_out, float* f_out, float dt, int check_every,
+                     nsph_step_report* report) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_step: NULL solver");
+    NSPH_CHECK(S->N == 0 || (q_in && qd_in && q_out && qd_out), NSPH_ERR_INVALID,
+               "nsph_solver_step: NULL state array");
+    NSPH_CHECK(dt > 0.f && std::isfinite(dt), NSPH_ERR_INVALID, "nsph_solver_step: dt must be positive");
+    NSPH_CHECK(S->N == 0 || (q_in != q_out && qd_in != qd_out), NSPH_ERR_INVALID,
+               "nsph_solver_step: state_in and state_out must not alias");
+    NSPH_CHECK(check_every >= 0, NSPH_ERR_INVALID, "nsph_solver_step: check_every must be >= 0");
+    NSPH_CHECK(S->warm != NSPH_WARM_MLP, NSPH_ERR_INVALID,
+               "nsph_solver_step: the MLP warm start needs nsph_solver_step_mlp");
+    hipStream_t st = S->ctx->stream;
+    const int Nf = S->Nf, W = S->W, C = S->C;
+    const float dt2 = (float)((double)dt * (double)dt);
+    SphDev D = make_dev(S);
+
+    hipLaunchKernelGGL(k_sph_step_init, dim3((W + 255) / 256), dim3(256), 0, st, D, W);
+    if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
+    if (Nf > 0) {
+        EventPair* ep_build;
+        NSPH_TRY(tic(S, KT_BUILD, &ep_build));
+        if (S->have_prev_keys)
+            hipLaunchKernelGGL(k_cells_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
+                               S->skeys, Nf, bp<int2>(S->ftable));
+        hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in,
+                           bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw), Nf,
+                           bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+        NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
+                            st, &S->skeys, &S->perm));
+        S->have_prev_keys = 1;
+        hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
+                           S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
+                           Nf, D.xs, D.vs, D.pws, D.ftable);
+        NSPH_TRY(toc(S, ep_build));
+        TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_neighbors, dim3(C), dim3(CHUNK), 0, st, D));
+        TIMED(KT_DENSITY, hipLaunchKernelGGL(k_sph_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
+        TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
+        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
+        NSPH_HIP(hipGetLastError());
+        const int red_blocks = (W * 64 + 255) / 256;
+        int launched = 0;
+        for (int it = 0; it < S->max_iters; ++it) {
+            TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C), dim3(CHUNK), 0, st, D));
+            TIMED(KT_PUPDATE, hipLaunchKernelGGL(k_sph_pupdate, dim3(C), dim3(CHUNK), 0, st, D, dt2,
+                                                 (float)S->omega));
+            TIMED(KT_REDUCE, hipLaunchKernelGGL(k_sph_reduce, dim3(red_blocks), dim3(256), 0, st, D, W,
+                                                it, S->tol, S->min_iters, S->max_iters));
+            launched = it + 1;
+            if (check_every > 0 && launched >= S->min_iters && (launched % check_every) == 0 &&
+                launched < S->max_iters) {
+                NSPH_HIP(hipMemcpyAsync(S->h_pinned, bp<int>(S->act) + (it & 1), 4,
+                                        hipMemcpyDeviceToHost, st));
+                NSPH_HIP(hipStreamSynchronize(st));
+                if (S->h_pinned[0] == 0) break;
+            }
+        }
+        S->last_iters_launched = launched;
+        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C), dim3(CHUNK), 0, st, D, S->perm,
+                                           bp<int>(S->fid), S->d_inv_mass, dt, S->v_max, q_out,
+                                           qd_out, f_out, bp<float>(S->pprev)));
+    }
+    if (S->No > 0) {
+        TIMED(KT_PASSTHROUGH,
+              hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
+                                 bp<int>(S->oid), S->No, q_in, qd_in, S->d_inv_mass, S->d_flags,
+                                 S->d_world, bp<PhysWorld>(S->phys), W, dt, S->v_max, q_out,
+                                 qd_out, f_out));
+    }
+    NSPH_HIP(hipGetLastError());
+    S->stepped = 1;
+    if (report) NSPH_TRY(collect_report(S, report));
+    return NSPH_OK;
+}
+
+int nsph_solver_report(nsph_solver* S, nsph_step_report* report) {
+    NSPH_CHECK(S && report, NSPH_ERR_INVALID, "nsph_solver_report: NULL argument");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_report: no step has run");
+    return collect_report(S, report);
+}
+
+int nsph_solver_fields(nsph_solver* S, float* d_rho, float* d_p, int* d_cnt) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_fields: NULL solver");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_fields: no step has run");
+    hipStream_t st = S->ctx->stream;
+    const size_t N = (size_t)S->N;
+    if (N == 0) return NSPH_OK;
+    if (d_rho) hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_rho, N);
+    if (d_p) hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_p, N);
+    if (d_cnt) hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_cnt, N, 0);
+    if (S->Nf > 0)
+        hipLaunchKernelGGL(k_sph_fields, dim3((S->Nf + 255) / 256), dim3(256), 0, st, S->perm,
+                           bp<int>(S->fid), S->Nf, bp<float>(S->rho), bp<float>(S->p),
+                           bp<int>(S->ncnt), d_rho, d_p, d_cnt);
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+int nsph_solver_world_stats(nsph_solver* S, int* h_iters, double* h_err, int* h_state) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_world_stats: NULL solver");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_world_stats: no step has run");
+    hipStream_t st = S->ctx->stream;
+    const size_t W = (size_t)S->W;
+    if (h_iters) NSPH_HIP(hipMemcpyAsync(h_iters, S->iters.ptr, W * 4, hipMemcpyDeviceToHost, st));
+    if (h_err) NSPH_HIP(hipMemcpyAsync(h_err, S->err.ptr, W * 8, hipMemcpyDeviceToHost, st));
+    if (h_state) NSPH_HIP(hipMemcpyAsync(h_state, S->conv.ptr, W * 4, hipMemcpyDeviceToHost, st));
+    NSPH_HIP(hipStreamSynchronize(st));
+    return NSPH_OK;
+}
+
+int nsph_solver_neighbors(nsph_solver* S, int* d_counts, int* d_idx) {
+    NSPH_CHECK(S && d_counts && d_idx, NSPH_ERR_INVALID, "nsph_solver_neighbors: NULL argument");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_neighbors: no step has run");
+    hipStream_t st = S->ctx->stream;
+    const size_t N = (size_t)S->N;
+    if (N == 0) return NSPH_OK;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_counts, N, 0);
+    if (S->Nf > 0)
+        hipLaunchKernelGGL(k_sph_export_nbrs, dim3((S->Nf + 255) / 256), dim3(256), 0, st, S->perm,
+                           bp<int>(S->fid), S->Nf, S->N, S->cap, bp<int>(S->nbr), bp<int>(S->ncnt),
+                           d_counts, d_idx);
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+int nsph_solver_set_warm_start(nsph_solver* S, int mode) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_warm_start: NULL solver");
+    NSPH_CHECK(mode >= NSPH_WARM_ZERO && mode <= NSPH_WARM_MLP, NSPH_ERR_INVALID,
+               "nsph_solver_set_warm_start: unknown mode %d", mode);
+    S->warm = mode;
+    return NSPH_OK;
+}
+
+int nsph_solver_reset_pressure(nsph_solver* S) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_reset_pressure: NULL solver");
+    const size_t n = (size_t)S->Nf + 1;
+    hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S->ctx->stream,
+                       bp<float>(S->pprev), n);
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+int nsph_solver_num_fluid(const nsph_solver* S, int* n_fluid, int* n_boundary) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_num_fluid: NULL solver");
+    if (n_fluid) *n_fluid = S->Nf;
+    if (n_boundary) *n_boundary = S->Nb;
+    return NSPH_OK;
+}
+
+int nsph_solver_set_timing(nsph_solver* S, int enable) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_timing: NULL solver");
+    NSPH_TRY(flush_timing(S));
+    S->timing = enable ? 1 : 0;
+    for (int k = 0; k < KT_COUNT; ++k) {
+        S->t_ms[k] = 0.0;
+        S->t_count[k] = 0;
+    }
+    return NSPH_OK;
+}
+
+int nsph_solver_timing(nsph_solver* S, double* h_ms, int* h_launches) {
+    NSPH_CHECK(S && h_ms && h_launches, NSPH_ERR_INVALID, "nsph_solver_timing: NULL argument");
+    NSPH_TRY(flush_timing(S));
+    for (int k = 0; k < KT_COUNT; ++k) {
+        h_ms[k] = S->t_ms[k];
+        h_launches[k] = (int)S->t_count[k];
+    }
+    return NSPH_OK;
+}
+
+int nsph_kernel_count(void) { return KT_COUNT; }
+
+const char* nsph_kernel_name(int k) { return (k >= 0 && k < KT_COUNT) ? kKernelNames[k] : ""; }
+
+int nsph_solver_iterations_launched(const nsph_solver* S, int* n) {
+    NSPH_CHECK(S && n, NSPH_ERR_INVALID, "nsph_solver_iterations_launched: NULL argument");
+    *n = S->last_iters_launched;
+    return NSPH_OK;
+}
+
+int nsph_solver_destroy(nsph_solver* S) {
+    if (!S) return NSPH_OK;
+    solver_free(S);
+    delete S;
+    return NSPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,315 @@
+"""Particle subset of Newton's sim data model on PyTorch-ROCm tensors.
+
+Mirrors the names, argument meaning and error behaviour of
+  ModelBuilder  newton/_src/sim/builder.py:795 (particles: 7186-7280, 8037-8137,
+                world scopes 2815-2907, finalize particle section 9815-9855, gravity 10655-10668)
+  Model         newton/_src/sim/model.py:140-187, 827-851
+  State         newton/_src/sim/state.py:101-108, 156-229
+  Control       newton/_src/sim/control.py:9
+  ParticleFlags newton/_src/geometry/flags.py:8-14
+Vectors are (N, 3) float32 tensors instead of wp.array(dtype=vec3).
+"""
+
+from __future__ import annotations
+
+import enum
+from collections.abc import Sequence
+from typing import Any
+
+import numpy as np
+import torch
+
+
+class ParticleFlags(enum.IntEnum):
+    """Flags for particle properties (newton/_src/geometry/flags.py:8-14)."""
+
+    ACTIVE = 1 << 0
+
+
+class Axis(enum.IntEnum):
+    X = 0
+    Y = 1
+    Z = 2
+
+
+def quat_to_matrix(q: Sequence[float]) -> np.ndarray:
+    """Rotation matrix of a unit quaternion given as (x, y, z, w), in float64."""
+    x, y, z, w = (float(c) for c in q)
+    return np.array(
+        [
+            [1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - z * w), 2.0 * (x * z + y * w)],
+            [2.0 * (x * y + z * w), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - x * w)],
+            [2.0 * (x * z - y * w), 2.0 * (y * z + x * w), 1.0 - 2.0 * (x * x + y * y)],
+        ],
+        dtype=np.float64,
+    )
+
+
+def _identity_quat(q: Sequence[float]) -> bool:
+    return tuple(float(c) for c in q) == (0.0, 0.0, 0.0, 1.0)
+
+
+class ModelBuilder:
+    """Builds a particle Model (particle methods of newton/_src/sim/builder.py)."""
+
+    def __init__(self, up_axis: Axis | int = Axis.Z, gravity: float = -9.81):
+        self.up_axis = Axis(int(up_axis))
+        self.gravity = float(gravity)
+        self.world_count = 0
+        self.world_gravity: list[tuple[float, float, float]] = []
+        self._current_world = -1
+        self.default_particle_radius = 0.1
+        self.particle_max_velocity = 1e5
+        # particle data in chunks (float64 until finalize, as the reference lists hold)
+        self._q: list[np.ndarray] = []
+        self._qd: list[np.ndarray] = []
+        self._mass: list[np.ndarray] = []
+        self._radius: list[np.ndarray] = []
+        self._flags: list[np.ndarray] = []
+        self._world: list[np.ndarray] = []
+        self._count = 0
+
+    # ---- worlds (builder.py:2815-2907) -------------------------------------------------
+    @property
+    def current_world(self) -> int:
+        return self._current_world
+
+    @property
+    def up_vector(self) -> tuple[float, float, float]:
+        v = [0.0, 0.0, 0.0]
+        v[int(self.up_axis)] = 1.0
+        return tuple(v)
+
+    def begin_world(self, label: str | None = None, attributes: dict | None = None, gravity=None):
+        if self._current_world != -1:
+            raise RuntimeError(
+                f"Cannot begin a new world: already in world context (current_world={self._current_world}). "
+                "Call end_world() first to close the current world context."
+            )
+        self._current_world = self.world_count
+        self.world_count += 1
+        if gravity is not None:
+            self.world_gravity.append(tuple(float(g) for g in gravity))
+        else:
+            u = self.up_vector
+            self.world_gravity.append((u[0] * self.gravity, u[1] * self.gravity, u[2] * self.gravity))
+
+    def end_world(self):
+        if self._current_world == -1:
+            raise RuntimeError("Cannot end world: not currently in a world context (current_world is already -1).")
+        self._current_world = -1
+
+    # ---- particles (builder.py:7186-7280, 8037-8137) ------------------------------------
+    @property
+    def particle_count(self) -> int:
+        return self._count
+
+    @property
+    def particle_q(self) -> np.ndarray:
+        return np.concatenate(self._q) if self._q else np.zeros((0, 3))
+
+    def _append(self, pos, vel, mass, radius, flags):
+        n = pos.shape[0]
+        self._q.append(np.asarray(pos, dtype=np.float64).reshape(n, 3))
+        self._qd.append(np.asarray(vel, dtype=np.float64).reshape(n, 3))
+        self._mass.append(np.asarray(mass, dtype=np.float64).reshape(n))
+        self._radius.append(np.asarray(radius, dtype=np.float64).reshape(n))
+        self._flags.append(np.asarray(flags, dtype=np.int64).reshape(n))
+        self._world.append(np.full(n, self._current_world, dtype=np.int32))
+        self._count += n
+
+    def add_particle(self, pos, vel, mass: float, radius: float | None = None,
+                     flags: int = ParticleFlags.ACTIVE, custom_attributes: dict | None = None) -> int:
+        if radius is None:
+            radius = self.default_particle_radius
+        self._append(np.array([pos], dtype=np.float64), np.array([vel], dtype=np.float64),
+                     [mass], [radius], [int(flags)])
+        return self._count - 1
+
+    def add_particles(self, pos, vel, mass, radius=None, flags=None, custom_attributes: dict | None = None):
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = np.asarray(vel, dtype=np.float64).reshape(n, 3)
+        mass = np.asarray(mass, dtype=np.float64).reshape(n)
+        if radius is None:
+            radius = np.full(n, self.default_particle_radius)
+        if flags is None:
+            flags = np.full(n, int(ParticleFlags.ACTIVE))
+        self._append(pos, vel, mass, np.asarray(radius, dtype=np.float64).reshape(n),
+                     np.asarray(flags, dtype=np.int64).reshape(n))
+
+    def add_particle_grid(self, pos, rot, vel, dim_x: int, dim_y: int, dim_z: int,
+                          cell_x: float, cell_y: float, cell_z: float, mass: float, jitter: float,
+                          radius_mean: float | None = None, radius_std: float = 0.0,
+                          flags: list[int] | int | None = None,
+                          custom_attributes: dict[str, Any] | None = None):
+        """Regular lattice of particles (builder.py:8037-8137).
+
+        Lattice order is numpy meshgrid's default 'xy' indexing: y slowest, then x, then z
+        fastest (builder.py:8084-8087). Rotation and translation in float64
+        (builder.py:8090-8091); jitter (rng.random - 0.5) * jitter from
+        numpy.random.default_rng(42 + particle_count) (builder.py:8095-8096); radii
+        radius_mean (+ standard_normal * radius_std) from the same generator (8098-8103).
+        """
+        px = np.arange(dim_x) * cell_x
+        py = np.arange(dim_y) * cell_y
+        pz = np.arange(dim_z) * cell_z
+        points = np.stack(np.meshgrid(px, py, pz)).reshape(3, -1).T
+        rot_mat = np.eye(3) if _identity_quat(rot) else quat_to_matrix(rot)
+        points = points @ rot_mat.reshape(3, 3).T + np.array(pos, dtype=np.float64)
+        velocity = np.broadcast_to(np.array(vel, dtype=np.float64).reshape(1, 3), points.shape)
+        rng = np.random.default_rng(42 + self._count)
+        points += (rng.random(points.shape) - 0.5) * jitter
+        if radius_mean is None:
+            radius_mean = self.default_particle_radius
+        radii = np.full(points.shape[0], fill_value=radius_mean, dtype=np.float64)
+        if radius_std > 0.0:
+            radii += rng.standard_normal(radii.shape) * radius_std
+        n = points.shape[0]
+        masses = np.full(n, mass, dtype=np.float64)
+        fl = np.full(n, int(ParticleFlags.ACTIVE) if flags is None else 0, dtype=np.int64)
+        if flags is not None:
+            fl[:] = int(flags) if np.isscalar(flags) else np.asarray(flags)
+        self.add_particles(points, velocity, masses, radii, fl)
+
+    # ---- finalize (builder.py:9745, 9815-9855, 10655-10668) -----------------------------
+    def finalize(self, device: str | torch.device | None = None, requires_grad: bool = False) -> Model:
+        if device is None:
+            device = "cuda:0" if torch.cuda.is_available() else "cpu"
+        device = torch.device(device)
+        m = Model(device)
+        n = self._count
+        q = np.concatenate(self._q) if n else np.zeros((0, 3))
+        qd = np.concatenate(self._qd) if n else np.zeros((0, 3))
+        mass = np.concatenate(self._mass).astype(np.float32) if n else np.zeros(0, np.float32)
+        radius = np.concatenate(self._radius) if n else np.zeros(0)
+        flags = np.concatenate(self._flags) if n else np.zeros(0, np.int64)
+        world = np.concatenate(self._world) if n else np.zeros(0, np.int32)
+        inv_mass = np.divide(1.0, mass, out=np.zeros_like(mass), where=mass != 0.0)
+        W = max(self.world_count, 1)
+        m.world_count = self.world_count
+        m.particle_q = torch.from_numpy(q.astype(np.float32)).to(device)
+        m.particle_qd = torch.from_numpy(qd.astype(np.float32)).to(device)
+        m.particle_mass = torch.from_numpy(mass).to(device)
+        m.particle_inv_mass = torch.from_numpy(inv_mass.astype(np.float32)).to(device)
+        m.particle_radius = torch.from_numpy(radius.astype(np.float32)).to(device)
+        m.particle_flags = torch.from_numpy(flags.astype(np.int32)).to(device)
+        m.particle_world = torch.from_numpy(world.astype(np.int32)).to(device)
+        m.particle_max_radius = float(np.max(radius)) if n else 0.0
+        m.particle_max_velocity = float(self.particle_max_velocity)
+        m.particle_world_start = torch.from_numpy(_world_starts(world, self.world_count)).to(device)
+        if self.world_gravity:
+            g = np.array(self.world_gravity, dtype=np.float32)
+        else:
+            u = self.up_vector
+            g = np.array([[u[0] * self.gravity, u[1] * self.gravity, u[2] * self.gravity]] * W,
+                         dtype=np.float32)
+        m.gravity = torch.from_numpy(g.reshape(-1, 3)).to(device)
+        m.up_axis = self.up_axis
+        if n > 1 and m.particle_max_radius > 0.0:
+            from .hashgrid import HashGrid
+
+            m.particle_grid = HashGrid(128, 128, 128, device=device)
+        else:
+            m.particle_grid = None
+        return m
+
+
+def _world_starts(world: np.ndarray, world_count: int) -> np.ndarray:
+    """particle_world_start of builder.py:9545-9604: [start_0 .. start_{W-1}, global tail, total]."""
+    n = world.shape[0]
+    front = 0
+    for w in world:
+        if w == -1:
+            front += 1
+        else:
+            break
+    starts = np.zeros(world_count + 2, dtype=np.int32)
+    counts = np.bincount(world[world >= 0], minlength=world_count) if world_count else np.zeros(0, int)
+    acc = front
+    for w in range(world_count):
+        starts[w] = acc
+        acc += int(counts[w])
+    starts[world_count] = acc
+    starts[world_count + 1] = n
+    return starts
+
+
+class Model:
+    """Particle part of newton.Model (newton/_src/sim/model.py:140-187)."""
+
+    def __init__(self, device: torch.device | str = "cpu"):
+        self.device = torch.device(device)
+        self.requires_grad = False
+        self.world_count = 0
+        self.particle_q = None
+        self.particle_qd = None
+        self.particle_mass = None
+        self.particle_inv_mass = None
+        self.particle_radius = None
+        self.particle_max_radius = 0.0
+        self.particle_ke = 1.0e3
+        self.particle_kd = 1.0e2
+        self.particle_kf = 1.0e2
+        self.particle_mu = 0.5
+        self.particle_cohesion = 0.0
+        self.particle_adhesion = 0.0
+        self.particle_grid = None
+        self.particle_flags = None
+        self.particle_max_velocity = 1e5
+        self.particle_world = None
+        self.particle_world_start = None
+        self.gravity = None
+        self.up_axis = Axis.Z
+        self.body_count = 0
+
+    @property
+    def particle_count(self) -> int:
+        return 0 if self.particle_q is None else int(self.particle_q.shape[0])
+
+    def state(self, requires_grad: bool | None = None) -> State:
+        s = State()
+        if self.particle_count:
+            s.particle_q = self.particle_q.clone()
+            s.particle_qd = self.particle_qd.clone()
+            s.particle_f = torch.zeros_like(self.particle_qd)
+        return s
+
+    def control(self, requires_grad: bool | None = None, clone_variables: bool = True) -> Control:
+        return Control()
+
+    def contacts(self) -> Contacts:
+        return Contacts()
+
+
+class State:
+    """Particle part of newton.State (newton/_src/sim/state.py:101-108, 156-229)."""
+
+    def __init__(self) -> None:
+        self.particle_q: torch.Tensor | None = None
+        self.particle_qd: torch.Tensor | None = None
+        self.particle_f: torch.Tensor | None = None
+
+    @property
+    def particle_count(self) -> int:
+        return 0 if self.particle_q is None else int(self.particle_q.shape[0])
+
+    def clear_forces(self) -> None:
+        if self.particle_count:
+            self.particle_f.zero_()
+
+    def assign(self, other: State) -> None:
+        for name in ("particle_q", "particle_qd", "particle_f"):
+            dst, src = getattr(self, name), getattr(other, name)
+            if (dst is None) != (src is None):
+                raise ValueError(f"State array '{name}' is present in only one of the two states.")
+            if dst is not None:
+                dst.copy_(src)
+
+
+class Control:
+    """newton.Control (no particle controls are used by the SPH step)."""
+
+
+class Contacts:
+    """newton.Contacts placeholder: the SPH step handles walls as boundary particles."""

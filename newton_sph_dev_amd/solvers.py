@@ -1,0 +1,306 @@
+"""Solver API: SolverBase (newton/_src/solvers/solver.py:177-366) and SolverSPH.
+
+SolverSPH keeps the reference contract ``step(state_in, state_out, control, contacts, dt)``
+(solver.py:301-316) and runs the whole particle step in libnsph (gfx950 HIP kernels).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from collections.abc import Sequence
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .sim import Contacts, Control, Model, ModelBuilder, State
+
+
+class SolverNotifyFlags(enum.IntFlag):
+    """newton/_src/solvers/flags.py"""
+
+    JOINT_PROPERTIES = 1 << 0
+    JOINT_DOF_PROPERTIES = 1 << 1
+    BODY_PROPERTIES = 1 << 2
+    BODY_INERTIAL_PROPERTIES = 1 << 3
+    SHAPE_PROPERTIES = 1 << 4
+    MODEL_PROPERTIES = 1 << 5
+    CONSTRAINT_PROPERTIES = 1 << 6
+    TENDON_PROPERTIES = 1 << 7
+    ACTUATOR_PROPERTIES = 1 << 8
+
+
+class SolverBase:
+    """Generic base class for solvers (solver.py:177-366)."""
+
+    def __init__(self, model: Model):
+        self.model = model
+
+    @property
+    def device(self) -> torch.device:
+        return self.model.device
+
+    def step(self, state_in: State, state_out: State, control: Control | None,
+             contacts: Contacts | None, dt: float) -> None:
+        raise NotImplementedError()
+
+    def notify_model_changed(self, flags: int) -> None:
+        pass
+
+    def update_contacts(self, contacts: Contacts, state: State | None = None) -> None:
+        raise NotImplementedError()
+
+    @classmethod
+    def register_custom_attributes(cls, builder: ModelBuilder) -> None:
+        pass
+
+
+_WARM = {"zero": N.WARM_ZERO, "prev": N.WARM_PREV, "mlp": N.WARM_MLP}
+
+
+def _per_world(value, W: int, name: str) -> np.ndarray:
+    arr = np.asarray(value, dtype=np.float64).reshape(-1)
+    if arr.size == 1:
+        arr = np.full(W, float(arr[0]))
+    if arr.size != W:
+        raise ValueError(f"{name} needs 1 or {W} values, got {arr.size}")
+    return arr
+
+
+@dataclass
+class SPHParams:
+    """Resolved SolverSPH parameters (host side, no GPU needed)."""
+
+    n_worlds: int
+    smoothing_length: np.ndarray
+    boundary_volume: np.ndarray
+    domain_lo: np.ndarray  # (W, 3)
+    domain_hi: np.ndarray
+    rest_density: float
+    viscosity: float
+    omega: float
+    tolerance: float
+    min_iters: int
+    max_iters: int
+    max_neighbors: int
+    warm_start: str
+
+    @staticmethod
+    def resolve(model: Model, smoothing_length, *, rest_density: float = 1000.0,
+                viscosity: float = 1.0e-4, boundary_volume=None, domain=None, omega: float = 0.5,
+                tolerance: float = 1.0e-3, min_iters: int = 2, max_iters: int = 100,
+                max_neighbors: int = 64, warm_start: str = "zero") -> "SPHParams":
+        if warm_start not in _WARM:
+            raise ValueError(f"warm_start must be one of {sorted(_WARM)}")
+        W = max(int(model.world_count), 1)
+        h = _per_world(smoothing_length, W, "smoothing_length")
+        if np.any(h <= 0):
+            raise ValueError("smoothing_length must be positive")
+        vb = _per_world((h * 0.5) ** 3 if boundary_volume is None else boundary_volume, W,
+                        "boundary_volume")
+        if domain is not None:
+            lo = np.asarray(domain[0], dtype=np.float64).reshape(-1, 3)
+            hi = np.asarray(domain[1], dtype=np.float64).reshape(-1, 3)
+            if lo.shape[0] == 1:
+                lo = np.repeat(lo, W, axis=0)
+                hi = np.repeat(hi, W, axis=0)
+            if lo.shape != (W, 3) or hi.shape != (W, 3):
+                raise ValueError("domain must be (lo, hi) with 1 or world_count boxes")
+        else:
+            lo = np.zeros((W, 3))
+            hi = np.ones((W, 3))
+            if model.particle_count:
+                q = model.particle_q.detach().cpu().numpy().astype(np.float64)
+                wd = model.particle_world.detach().cpu().numpy()
+                wd = np.where(wd < 0, 0, wd)
+                order = np.argsort(wd, kind="stable")
+                bounds = np.searchsorted(wd[order], np.arange(W + 1))
+                for w in range(W):
+                    sel = q[order[bounds[w]:bounds[w + 1]]]
+                    if sel.shape[0]:
+                        pad = 2.0 * h[w]
+                        lo[w] = sel.min(axis=0) - pad
+                        hi[w] = sel.max(axis=0) + pad
+        return SPHParams(W, h, vb, lo, hi, float(rest_density), float(viscosity), float(omega),
+                         float(tolerance), int(min_iters), int(max_iters), int(max_neighbors),
+                         warm_start)
+
+
+class SolverSPH(SolverBase):
+    """Incompressible SPH (IISPH) particle step on MI355X.
+
+    Fluid particles are ACTIVE with mass > 0; ACTIVE zero-mass particles are static
+    boundary samples (walls, obstacles) with pseudo-mass rest_density * boundary_volume;
+    inactive particles are carried unchanged (solver.py:27-29).  Worlds never interact.
+
+    Args:
+        model: the Model (on a ROCm device).
+        smoothing_length: kernel support radius H [m], scalar or one per world.
+        rest_density: rho_0 [kg/m^3].
+        viscosity: kinematic viscosity nu [m^2/s] of the explicit SPH viscosity term.
+        boundary_volume: rest volume of one boundary particle, default (H/2)^3.
+        domain: (lo, hi) box per world for the cell tables (particles outside still work,
+            they are clamped into border cells); default: bounding box of the world's
+            particles at creation, padded by 2 H.
+        omega: relaxed Jacobi factor. tolerance: mean compression / rho_0 stop test.
+        min_iters, max_iters: Jacobi iteration bounds (the step never stops silently at
+            max_iters: `last_report["unconverged_worlds"]` counts such worlds).
+        max_neighbors: capacity of the fluid-fluid neighbour list per particle.
+        warm_start: "zero", "prev" (0.5 * previous pressure) or "mlp".
+        check_every: host convergence check interval in iterations; 0 launches max_iters
+            early-exit iterations with no host read.
+        report: collect a step report (one host sync per step) into `last_report`.
+    """
+
+    def __init__(self, model: Model, smoothing_length: float | Sequence[float], *,
+                 rest_density: float = 1000.0, viscosity: float = 1.0e-4,
+                 boundary_volume: float | Sequence[float] | None = None,
+                 domain: tuple | None = None, omega: float = 0.5, tolerance: float = 1.0e-3,
+                 min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 64,
+                 warm_start: str = "zero", check_every: int = 1, report: bool = True):
+        super().__init__(model)
+        if model.device.type != "cuda":
+            raise RuntimeError("SolverSPH runs on a ROCm GPU only (model.device is %s)" % model.device)
+        prm = SPHParams.resolve(model, smoothing_length, rest_density=rest_density,
+                                viscosity=viscosity, boundary_volume=boundary_volume, domain=domain,
+                                omega=omega, tolerance=tolerance, min_iters=min_iters,
+                                max_iters=max_iters, max_neighbors=max_neighbors,
+                                warm_start=warm_start)
+        self.params = prm
+        self.W = prm.n_worlds
+        for name in ("smoothing_length", "boundary_volume", "rest_density", "viscosity", "omega",
+                     "tolerance", "min_iters", "max_iters", "max_neighbors", "warm_start",
+                     "domain_lo", "domain_hi"):
+            setattr(self, name, getattr(prm, name))
+        self.check_every = int(check_every)
+        self.report = bool(report)
+        self.last_report: dict | None = None
+        self._handle = None
+        self._create()
+
+    # ---- set-up ---------------------------------------------------------------------------
+    def _create(self):
+        m = self.model
+        dev = m.device.index or 0
+        self._ctx = N.context(dev)
+        N.set_stream(self._ctx, torch.cuda.current_stream(m.device).cuda_stream)
+        W = self.W
+        g = m.gravity.detach().cpu().numpy().astype(np.float32).reshape(-1, 3)
+        if g.shape[0] < W:
+            g = np.concatenate([g, np.repeat(g[:1], W - g.shape[0], axis=0)])
+        self._gravity = np.ascontiguousarray(g[:W])
+        self._h = np.ascontiguousarray(self.smoothing_length)
+        self._vb = np.ascontiguousarray(self.boundary_volume)
+        self._lo = np.ascontiguousarray(self.domain_lo.reshape(-1))
+        self._hi = np.ascontiguousarray(self.domain_hi.reshape(-1))
+        md = N.ModelDesc()
+        md.n_particles = m.particle_count
+        md.n_worlds = W
+        md.d_q = N.ptr(m.particle_q)
+        md.d_qd = N.ptr(m.particle_qd)
+        md.d_mass = N.ptr(m.particle_mass)
+        md.d_inv_mass = N.ptr(m.particle_inv_mass)
+        md.d_flags = N.ptr(m.particle_flags)
+        md.d_world = N.ptr(m.particle_world)
+        md.h_gravity = self._gravity.ctypes.data_as(C.POINTER(C.c_float))
+        md.v_max = float(m.particle_max_velocity)
+        pr = N.SphParams()
+        pr.h_support = self._h.ctypes.data_as(C.POINTER(C.c_double))
+        pr.h_boundary_volume = self._vb.ctypes.data_as(C.POINTER(C.c_double))
+        pr.h_domain_lo = self._lo.ctypes.data_as(C.POINTER(C.c_double))
+        pr.h_domain_hi = self._hi.ctypes.data_as(C.POINTER(C.c_double))
+        pr.rest_density = self.rest_density
+        pr.viscosity = self.viscosity
+        pr.omega = self.omega
+        pr.tolerance = self.tolerance
+        pr.min_iters = self.min_iters
+        pr.max_iters = self.max_iters
+        pr.max_neighbors = self.max_neighbors
+        pr.warm_start = _WARM[self.warm_start] if self.warm_start != "mlp" else N.WARM_ZERO
+        h = C.c_void_p()
+        N.call("nsph_solver_create", self._ctx, C.byref(md), C.byref(pr), C.byref(h))
+        self._handle = h
+        nf, nb = C.c_int(), C.c_int()
+        N.call("nsph_solver_num_fluid", h, C.byref(nf), C.byref(nb))
+        self.fluid_count = nf.value
+        self.boundary_count = nb.value
+
+    def notify_model_changed(self, flags: int) -> None:
+        """MODEL_PROPERTIES: re-create (re-classify particles, re-hash the static boundary)."""
+        if flags & SolverNotifyFlags.MODEL_PROPERTIES:
+            self._destroy()
+            self._create()
+
+    # ---- step ---------------------------------------------------------------------------------
+    def step(self, state_in: State, state_out: State, control: Control | None,
+             contacts: Contacts | None, dt: float) -> None:
+        m = self.model
+        if m.particle_count == 0:
+            return
+        N.set_stream(self._ctx, torch.cuda.current_stream(m.device).cuda_stream)
+        for t in (state_in.particle_q, state_in.particle_qd, state_out.particle_q, state_out.particle_qd):
+            if t is None or t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous():
+                raise ValueError("state arrays must be contiguous float32 tensors on the model device")
+        rep = N.StepReport() if self.report else None
+        N.call("nsph_solver_step", self._handle, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
+               N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(state_out.particle_f),
+               float(dt), self.check_every, C.byref(rep) if rep is not None else None)
+        if rep is not None:
+            self.last_report = rep.as_dict()
+            if rep.status == N.NSPH_ERR_CAPACITY:
+                raise N.CapacityError(rep.status, f"a neighbour list exceeded max_neighbors={self.max_neighbors} "
+                                                  f"(max count {rep.max_neighbor_count})")
+            if rep.status == N.NSPH_ERR_NONFINITE:
+                raise N.NsphError(rep.status, "non-finite position or velocity after the step")
+
+    def collect_report(self) -> dict:
+        rep = N.StepReport()
+        N.call("nsph_solver_report", self._handle, C.byref(rep))
+        self.last_report = rep.as_dict()
+        return self.last_report
+
+    # ---- inspection -----------------------------------------------------------------------------
+    def fields(self) -> dict[str, torch.Tensor]:
+        """Density, pressure and fluid-fluid neighbour count of the last step, particle order."""
+        n = self.model.particle_count
+        dev = self.model.device
+        rho = torch.empty(n, dtype=torch.float32, device=dev)
+        p = torch.empty(n, dtype=torch.float32, device=dev)
+        cnt = torch.empty(n, dtype=torch.int32, device=dev)
+        N.call("nsph_solver_fields", self._handle, N.ptr(rho), N.ptr(p), N.ptr(cnt))
+        return {"density": rho, "pressure": p, "neighbor_count": cnt}
+
+    def neighbors(self) -> tuple[torch.Tensor, torch.Tensor]:
+        """Fluid-fluid neighbour lists of the last step: counts[N], idx[max_neighbors, N] (ids)."""
+        n = self.model.particle_count
+        dev = self.model.device
+        counts = torch.empty(n, dtype=torch.int32, device=dev)
+        idx = torch.full((self.max_neighbors, n), -1, dtype=torch.int32, device=dev)
+        N.call("nsph_solver_neighbors", self._handle, N.ptr(counts), N.ptr(idx))
+        return counts, idx
+
+    def world_stats(self) -> dict[str, np.ndarray]:
+        W = self.W
+        it = np.zeros(W, dtype=np.int32)
+        err = np.zeros(W, dtype=np.float64)
+        st = np.zeros(W, dtype=np.int32)
+        N.call("nsph_solver_world_stats", self._handle, it.ctypes.data_as(C.POINTER(C.c_int)),
+               err.ctypes.data_as(C.POINTER(C.c_double)), st.ctypes.data_as(C.POINTER(C.c_int)))
+        return {"iterations": it, "density_error": err, "state": st}
+
+    def reset_pressure(self) -> None:
+        N.call("nsph_solver_reset_pressure", self._handle)
+
+    def _destroy(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and N._lib is not None:
+            try:
+                N._lib.nsph_solver_destroy(h)
+            except Exception:
+                pass
+        self._handle = None
+
+    def __del__(self):
+        self._destroy()

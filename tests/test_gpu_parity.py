@@ -1,0 +1,181 @@
+"""GPU parity of the SPH step and the HashGrid against the CPU oracle (through the C ABI).
+
+Bars:
+  - neighbour lists: set-equal to O(N^2) brute force under the FP32 predicate, and equal in
+    order to the oracle's cell-order restatement;
+  - every per-particle output (x, v, f, rho, p) and the iteration count: bit-exact against
+    the same-order FP32 restatement (IEEE + - * / sqrt, no contraction on both sides);
+  - against the FP64 oracle run with the same iteration count: relative error
+    <= 1e-5 on density (per particle) and <= 1e-5 on force (max-norm relative, i.e.
+    max|F - F64| / max|F64|), pressure <= 1e-5 max-norm relative.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from newton_sph_dev_amd import HashGrid, SolverSPH, scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RHO_TOL = 1e-5
+F_TOL = 1e-5
+P_TOL = 1e-5
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _lists_as_sets(counts, idx, slot_major: bool):
+    counts = np.asarray(counts)
+    out = []
+    for i in range(counts.shape[0]):
+        c = int(counts[i])
+        row = idx[:c, i] if slot_major else idx[i, :c]
+        out.append(tuple(sorted(int(v) for v in row)))
+    return out
+
+
+def _solver_and_model(cfg, **kw):
+    b = scenes.build_dam_break(cfg)
+    m = b.finalize("cuda:0")
+    s = SolverSPH(m, cfg.support, **kw)
+    return m, s
+
+
+def test_hashgrid_neighbors_match_bruteforce():
+    _print("hashgrid: 6000 random points, radius 0.05, dims 16^3 (periodic)")
+    rng = np.random.default_rng(3)
+    pts = rng.random((6000, 3)).astype(np.float32) * np.float32(0.9)
+    x = torch.from_numpy(pts).cuda()
+    g = HashGrid(16, 16, 16, device="cuda:0")
+    g.reserve(6000)
+    g.build(x, 0.05)
+    ids = g.point_ids().cpu().numpy()
+    assert np.array_equal(np.sort(ids), np.arange(6000))
+    counts, idx = g.neighbors(0.05, max_neighbors=96)
+    counts, idx = counts.cpu().numpy(), idx.cpu().numpy()
+    bc, bi = O.neighbors_brute(pts, None, np.float32(0.05) * np.float32(0.05), cap=96)
+    assert np.array_equal(counts, bc)
+    assert _lists_as_sets(counts, idx, True) == _lists_as_sets(bc, bi, False)
+    # queries at arbitrary points
+    qp = (rng.random((500, 3)) * 0.9).astype(np.float32)
+    qc, qi = g.query(torch.from_numpy(qp).cuda(), 0.05, max_neighbors=96)
+    qc, qi = qc.cpu().numpy(), qi.cpu().numpy()
+    d2 = ((qp[:, None, :] - pts[None, :, :]) ** 2)
+    d2 = (d2[..., 0] + d2[..., 1]) + d2[..., 2]
+    for k in range(500):
+        exp = tuple(np.nonzero(d2[k] < np.float32(0.05) * np.float32(0.05))[0].tolist())
+        assert tuple(sorted(qi[: qc[k], k].tolist())) == exp
+
+
+def test_hashgrid_rebuild_and_aliasing():
+    """Rebuild with fewer points (stale cells cleared); tiny periodic grid forces aliasing."""
+    _print("hashgrid: rebuild 3000 -> 1000 points, dims 3^3")
+    rng = np.random.default_rng(5)
+    g = HashGrid(3, 3, 3, device="cuda:0")
+    for n in (3000, 1000):
+        pts = (rng.random((n, 3)) * 2.0).astype(np.float32)
+        g.build(torch.from_numpy(pts).cuda(), 0.1)
+        counts, idx = g.neighbors(0.1, max_neighbors=64)
+        bc, bi = O.neighbors_brute(pts, None, np.float32(0.1) * np.float32(0.1), cap=64)
+        assert _lists_as_sets(counts.cpu().numpy(), idx.cpu().numpy(), True) == _lists_as_sets(bc, bi, False)
+
+
+@pytest.mark.parametrize("dims", [(12, 10, 14), (20, 16, 24)])
+def test_step_small_dam_break_parity(dims):
+    cfg = scenes.small_dam_break(*dims)
+    _print(f"dam break {dims}: one step, parity vs FP32 restatement and FP64 oracle")
+    m, s = _solver_and_model(cfg)
+    st0, st1 = m.state(), m.state()
+    s.step(st0, st1, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    rep = s.last_report
+    f = s.fields()
+    counts, idx = s.neighbors()
+    g = {k: v.cpu().numpy() for k, v in f.items()}
+    q1, qd1, f1 = (t.cpu().numpy() for t in (st1.particle_q, st1.particle_qd, st1.particle_f))
+    counts, idx = counts.cpu().numpy(), idx.cpu().numpy()
+
+    pb = O.problem_from_model(m, s)
+    q0, qd0 = pb.q0, pb.qd0
+    # neighbour sets vs brute force over fluid particles (fluid = ACTIVE and mass > 0)
+    fluid = ((pb.flags & 1) != 0) & (pb.inv_mass > 0)
+    fid = np.nonzero(fluid)[0]
+    H2 = np.float32(cfg.support * cfg.support)
+    bc, bi = O.neighbors_brute(q0[fid], None, H2, cap=96)
+    gpu_sets = _lists_as_sets(counts[fid], idx[:, fid], True)
+    brute = [tuple(sorted(int(fid[j]) for j in bi[k, : bc[k]])) for k in range(fid.size)]
+    assert gpu_sets == brute
+    # neighbour order vs the oracle's cell-order restatement
+    oc, oi, _ = O.solver_neighbors(pb, q0)
+    assert np.array_equal(oc, counts)
+    for i in fid:
+        assert np.array_equal(oi[: oc[i], i], idx[: counts[i], i])
+
+    r32 = O.step(pb, q0, qd0, precision="f32")
+    assert int(r32["iters"][0]) == rep["max_iterations"]
+    assert np.array_equal(r32["rho"], g["density"])
+    assert np.array_equal(r32["p"], g["pressure"])
+    assert np.array_equal(r32["q"], q1)
+    assert np.array_equal(r32["qd"], qd1)
+    assert np.array_equal(r32["f"], f1)
+
+    r64 = O.step(pb, q0, qd0, precision="f64", forced_iters=np.array([rep["max_iterations"]]))
+    rho_rel = np.abs(g["density"][fid] - r64["rho"][fid]) / np.abs(r64["rho"][fid])
+    f_rel = np.abs(f1[fid] - r64["f"][fid]).max() / np.abs(r64["f"][fid]).max()
+    p_rel = np.abs(g["pressure"][fid] - r64["p"][fid]).max() / max(np.abs(r64["p"][fid]).max(), 1e-30)
+    _print(f"  iters={rep['max_iterations']} rho_rel_max={rho_rel.max():.2e} f_rel={f_rel:.2e} p_rel={p_rel:.2e}")
+    assert rho_rel.max() <= RHO_TOL
+    assert f_rel <= F_TOL
+    assert p_rel <= P_TOL
+
+
+def test_multistep_bitexact_and_deterministic():
+    cfg = scenes.small_dam_break(10, 8, 12)
+    steps = 25
+    _print(f"dam break 10x8x12: {steps} steps twice (determinism) and vs FP32 restatement")
+    runs = []
+    for _ in range(2):
+        m, s = _solver_and_model(cfg, warm_start="prev")
+        a, b = m.state(), m.state()
+        for _k in range(steps):
+            s.step(a, b, None, None, scenes.DT)
+            a, b = b, a
+        torch.cuda.synchronize()
+        runs.append((a.particle_q.cpu().numpy(), a.particle_qd.cpu().numpy()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    pb = O.problem_from_model(m, s)
+    q, qd, pp = pb.q0, pb.qd0, None
+    for _k in range(steps):
+        r = O.step(pb, q, qd, pprev=pp, precision="f32")
+        q, qd, pp = r["q"], r["qd"], r["pprev"]
+    assert np.array_equal(q, runs[0][0])
+    assert np.array_equal(qd, runs[0][1])
+
+
+def test_dam_break_32k_one_step():
+    cfg = scenes.DAM_BREAK_32K
+    _print("dam break 32,768 fluid: one step parity")
+    m, s = _solver_and_model(cfg)
+    st0, st1 = m.state(), m.state()
+    s.step(st0, st1, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    rep = s.last_report
+    g = {k: v.cpu().numpy() for k, v in s.fields().items()}
+    f1 = st1.particle_f.cpu().numpy()
+    q1 = st1.particle_q.cpu().numpy()
+    pb = O.problem_from_model(m, s)
+    r32 = O.step(pb, pb.q0, pb.qd0, precision="f32")
+    assert np.array_equal(r32["q"], q1)
+    assert np.array_equal(r32["rho"], g["density"])
+    r64 = O.step(pb, pb.q0, pb.qd0, precision="f64", forced_iters=np.array([rep["max_iterations"]]))
+    fid = np.nonzero(g["density"] > 0)[0]
+    rho_rel = (np.abs(g["density"][fid] - r64["rho"][fid]) / r64["rho"][fid]).max()
+    f_rel = np.abs(f1[fid] - r64["f"][fid]).max() / np.abs(r64["f"][fid]).max()
+    _print(f"  iters={rep['max_iterations']} rho_rel={rho_rel:.2e} f_rel={f_rel:.2e} nbr_max={rep['max_neighbor_count']}")
+    assert rho_rel <= RHO_TOL and f_rel <= F_TOL
