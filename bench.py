@@ -34,8 +34,7 @@ def algorithmic_bytes(nbar: float) -> dict[str, float]:
     L = 4.0 * nbar + 4.0
     return {
         "build": 28 + 3 * 20 + 92 + 12,
-        "neighbors": 16 + L,
-        "density": 16 + 24 + L,
+        "neighbors_density": 16 + 4 + 4 + 16 + L,
         "advect": 16 + 16 + 4 + 32 + L,
         "source": 16 + 16 + 16 + 4 + 4 + 12 + L,
         "pressure_acc": 16 + 4 + 16 + 16 + L,

@@ -13,8 +13,8 @@
 //   rho[s], aii[s], src[s], p[s], pr[s]=p/rho^2   float
 //   gb[s]   float4 (sum_b gradW_ib, boundary count)
 //   avisc[s], vadv[s], ap[s]  float4
-//   nbr[k*Nf + s] int  fluid-fluid neighbour list, slot-major (lanes of a wave read
-//                      consecutive addresses for the same slot)
+//   nbr4[q*Nf + s] int4  fluid-fluid neighbour list, slots 4q..4q+3 of particle s (lanes
+//                        of a wave read 64 consecutive quads)
 // Boundary particles are hashed once into their own table (bxs, btable); only the density
 // pass visits them, and it folds them into gb, so the other passes never touch them.
 //
@@ -112,7 +112,7 @@ struct SphDev {
     float* pr;
     float4* ap;
     float* pws;  // warm-start input in sorted order
-    int* nbr;
+    int4* nbr4;
     int* ncnt;
     // iteration control
     double* partial;
@@ -180,60 +180,111 @@ __global__ void k_sph_step_init(SphDev D, int W) {
     }
 }
 
-// Fluid-fluid neighbour lists: j != i, |x_i - x_j|^2 < H^2 (FP32 predicate of grid_visit).
-__global__ __launch_bounds__(CHUNK) void k_sph_neighbors(SphDev D) {
-    int w, s;
-    if (!chunk_slot(D, &w, &s)) return;
-    const GridWorld g = D.gw[w];
-    const PhysWorld P = D.phys[w];
-    const float4 xi = D.xs[s];
-    int c = 0;
-    const int Nf = D.Nf, cap = D.cap;
-    int* __restrict__ nbr = D.nbr;
-    grid_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
-               [&](int j, float, float, float, float) {
-                   if (j == s) return;
-                   if (c < cap) nbr[(size_t)c * Nf + s] = j;
-                   ++c;
-               });
-    D.ncnt[s] = c < cap ? c : cap;
-    if (c > cap) atomicOr(&D.flags[0], 1);
-    atomicMax(&D.flags[1], c);
+// ---- neighbour lists ---------------------------------------------------------------------
+//
+// Lists are int4 quads: nbr4[q * Nf + s] holds slots 4q .. 4q+3 of sorted particle s, so a
+// wave reads 64 consecutive 16-byte quads per load.  Gather passes fetch one quad, issue the
+// four neighbour loads together, then accumulate in slot order (the order is fixed; only the
+// loads are batched).
+
+// Visit the hashed points of one row range in ascending sorted index: the cells cx0..cx0+nx-1
+// of a row are consecutive keys, so their points form the range [lo, hi).
+__device__ inline int2 row_range(const int2* __restrict__ table, int row, int x0, int nx) {
+    int lo = 0x7fffffff, hi = 0;
+    for (int ix = 0; ix < nx; ++ix) {
+        const int2 se = table[row + x0 + ix];
+        if (se.x < se.y) {
+            lo = min(lo, se.x);
+            hi = max(hi, se.y);
+        }
+    }
+    return make_int2(lo, hi);
 }
 
-// Density (compensated), the diagonal a_ii and the boundary gradient sum.
-//   rho_i = m_i W(0) + sum_j m_j W_ij + psi sum_b W_ib                (Neumaier, list order,
-//                                                                       then boundary order)
+// Candidates j of the rows of the (clamped) query box around (qx,qy,qz), in the order z, y,
+// then ascending sorted index; calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).
+// Positions are fetched four at a time.
+template <class F>
+__device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table,
+                                  const float4* __restrict__ xs, float qx, float qy, float qz,
+                                  float r, float r2max, F f) {
+    const float rp = r + r * NSPH_RANGE_PAD;
+    int x0, nx, y0, ny, z0, nz;
+    axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
+    axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
+    axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
+    for (int iz = 0; iz < nz; ++iz) {
+        for (int iy = 0; iy < ny; ++iy) {
+            const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0];
+            const int2 rg = row_range(table, row, x0, nx);
+            for (int j = rg.x; j < rg.y; j += 4) {
+                float4 p[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) p[u] = xs[min(j + u, rg.y - 1)];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (j + u < rg.y) {
+                        const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
+                        const float d2 = (dx * dx + dy * dy) + dz * dz;
+                        if (d2 < r2max) f(j + u, dx, dy, dz, d2);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Neighbour lists fused with the density pass.
+//   list:  j != i, |x_i - x_j|^2 < H^2 (FP32 predicate), candidate order
+//   rho_i = m_i W(0) + sum_j m_j W_ij + psi sum_b W_ib     (Neumaier, list then boundary order)
 //   Gf = sum_j m_j gradW_ij,  S = sum_j m_j |gradW_ij|^2,  Gb = sum_b gradW_ib
 //   G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.
-__global__ __launch_bounds__(CHUNK) void k_sph_density(SphDev D, float dt2) {
+__global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot(D, &w, &s)) return;
-    const PhysWorld P = D.phys[w];
     const GridWorld g = D.gw[w];
+    const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
-    const int Nf = D.Nf;
+    const int Nf = D.Nf, cap = D.cap;
+    int4* __restrict__ out = D.nbr4 + s;
+    int c = 0;
+    int4 quad = make_int4(0, 0, 0, 0);
     float sum = xi.w * P.W0, cmp = 0.0f;
     float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
-    const int n = D.ncnt[s];
-    for (int k = 0; k < n; ++k) {
-        const int j = D.nbr[(size_t)k * Nf + s];
-        const float4 xj = D.xs[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        float W, gf;
-        kern_wg(P, r2, &W, &gf);
-        neumaier(sum, cmp, xj.w * W);
-        const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
-        gfx += xj.w * gx;
-        gfy += xj.w * gy;
-        gfz += xj.w * gz;
-        S += xj.w * dot3(gx, gy, gz, gx, gy, gz);
-    }
+    rows_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
+               [&](int j, float dx, float dy, float dz, float r2) {
+                   if (j == s) return;
+                   if (c < cap) {
+                       const int u = c & 3;
+                       if (u == 0) quad.x = j;
+                       else if (u == 1) quad.y = j;
+                       else if (u == 2) quad.z = j;
+                       else {
+                           quad.w = j;
+                           out[(size_t)(c >> 2) * Nf] = quad;
+                       }
+                   }
+                   ++c;
+                   const float mj = D.xs[j].w;
+                   float W, gf;
+                   kern_wg(P, r2, &W, &gf);
+                   neumaier(sum, cmp, mj * W);
+                   const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+                   gfx += mj * gx;
+                   gfy += mj * gy;
+                   gfz += mj * gz;
+                   S += mj * dot3(gx, gy, gz, gx, gy, gz);
+               });
+    const int cl = c < cap ? c : cap;
+    if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
+    D.ncnt[s] = cl;
+    if (c > cap) atomicOr(&D.flags[0], 1);
+    atomicMax(&D.flags[1], c);
+
     float gbx = 0.f, gby = 0.f, gbz = 0.f;
     int nb = 0;
-    grid_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+    rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                [&](int, float dx, float dy, float dz, float r2) {
                    float W, gf;
                    kern_wg(P, r2, &W, &gf);
@@ -251,6 +302,37 @@ __global__ __launch_bounds__(CHUNK) void k_sph_density(SphDev D, float dt2) {
     D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
 }
 
+// Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
+// (four entries at a time) and use(data, j) accumulates.
+template <class T, class Load, class Use>
+__device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) {
+    const int n = D.ncnt[s];
+    const int4* __restrict__ L = D.nbr4 + s;
+    const size_t stride = (size_t)D.Nf;
+    for (int k = 0; k < n; k += 4) {
+        const int4 q = L[(size_t)(k >> 2) * stride];
+        const int m = n - k;
+        const int j0 = q.x, j1 = m > 1 ? q.y : q.x, j2 = m > 2 ? q.z : q.x, j3 = m > 3 ? q.w : q.x;
+        const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
+        use(a0, j0);
+        if (m > 1) use(a1, j1);
+        if (m > 2) use(a2, j2);
+        if (m > 3) use(a3, j3);
+    }
+}
+
+struct XV {
+    float4 x, v;
+    float r;
+};
+struct XS {
+    float4 x;
+    float s;
+};
+struct XA {
+    float4 x, a;
+};
+
 // Viscosity and advected velocity:
 //   a_visc = 10 nu sum_j (m_j/rho_j) ((v_ij . x_ij) / (r^2 + 0.01 H^2)) gradW_ij
 //   v_adv  = v + dt (g + a_visc)
@@ -260,23 +342,19 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
-    const int Nf = D.Nf;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    const int n = D.ncnt[s];
-    for (int k = 0; k < n; ++k) {
-        const int j = D.nbr[(size_t)k * Nf + s];
-        const float4 xj = D.xs[j];
-        const float4 vj = D.vs[j];
-        const float rj = D.rho[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const float gf = kern_gf(P, r2);
-        const float vx = dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, dx, dy, dz);
-        const float c = (xj.w / rj) * (vx / (r2 + P.eps2));
-        ax += c * (gf * dx);
-        ay += c * (gf * dy);
-        az += c * (gf * dz);
-    }
+    for_each_nbr<XV>(
+        D, s, [&](int j) { return XV{D.xs[j], D.vs[j], D.rho[j]}; },
+        [&](const XV& o, int) {
+            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            const float vx = dot3(vi.x - o.v.x, vi.y - o.v.y, vi.z - o.v.z, dx, dy, dz);
+            const float c = (o.x.w / o.r) * (vx / (r2 + P.eps2));
+            ax += c * (gf * dx);
+            ay += c * (gf * dy);
+            az += c * (gf * dz);
+        });
     const float avx = P.visc10 * ax, avy = P.visc10 * ay, avz = P.visc10 * az;
     D.avisc[s] = make_float4(avx, avy, avz, 0.f);
     D.vadv[s] = make_float4(vi.x + dt * (P.g[0] + avx), vi.y + dt * (P.g[1] + avy),
@@ -292,18 +370,15 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const float4 vi = D.vadv[s];
-    const int Nf = D.Nf;
     float sum = 0.f;
-    const int n = D.ncnt[s];
-    for (int k = 0; k < n; ++k) {
-        const int j = D.nbr[(size_t)k * Nf + s];
-        const float4 xj = D.xs[j];
-        const float4 vj = D.vadv[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const float gf = kern_gf(P, r2);
-        sum += xj.w * dot3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z, gf * dx, gf * dy, gf * dz);
-    }
+    for_each_nbr<XA>(
+        D, s, [&](int j) { return XA{D.xs[j], D.vadv[j]}; },
+        [&](const XA& o, int) {
+            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            sum += o.x.w * dot3(vi.x - o.a.x, vi.y - o.a.y, vi.z - o.a.z, gf * dx, gf * dy, gf * dz);
+        });
     const float4 gb = D.gb[s];
     sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
     const float rho = D.rho[s];
@@ -318,22 +393,19 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
 // Pressure acceleration  a^p_i = -( sum_j m_j (pr_i + pr_j) gradW_ij + (psi pr_i) Gb_i ).
 __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int s, float4 xi,
                                  float* ox, float* oy, float* oz) {
-    const int Nf = D.Nf;
     const float pri = D.pr[s];
     float ax = 0.f, ay = 0.f, az = 0.f;
-    const int n = D.ncnt[s];
-    for (int k = 0; k < n; ++k) {
-        const int j = D.nbr[(size_t)k * Nf + s];
-        const float4 xj = D.xs[j];
-        const float prj = D.pr[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const float gf = kern_gf(P, r2);
-        const float c = xj.w * (pri + prj);
-        ax += c * (gf * dx);
-        ay += c * (gf * dy);
-        az += c * (gf * dz);
-    }
+    for_each_nbr<XS>(
+        D, s, [&](int j) { return XS{D.xs[j], D.pr[j]}; },
+        [&](const XS& o, int) {
+            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            const float c = o.x.w * (pri + o.s);
+            ax += c * (gf * dx);
+            ay += c * (gf * dy);
+            az += c * (gf * dz);
+        });
     const float4 gb = D.gb[s];
     const float cb = P.psi * pri;
     *ox = -(ax + cb * gb.x);
@@ -367,18 +439,15 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
         const PhysWorld P = D.phys[w];
         const float4 xi = D.xs[s];
         const float4 ai = D.ap[s];
-        const int Nf = D.Nf;
         float sum = 0.f;
-        const int n = D.ncnt[s];
-        for (int k = 0; k < n; ++k) {
-            const int j = D.nbr[(size_t)k * Nf + s];
-            const float4 xj = D.xs[j];
-            const float4 aj = D.ap[j];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const float gf = kern_gf(P, r2);
-            sum += xj.w * dot3(ai.x - aj.x, ai.y - aj.y, ai.z - aj.z, gf * dx, gf * dy, gf * dz);
-        }
+        for_each_nbr<XA>(
+            D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
+            [&](const XA& o, int) {
+                const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+                const float r2 = (dx * dx + dy * dy) + dz * dz;
+                const float gf = kern_gf(P, r2);
+                sum += o.x.w * dot3(ai.x - o.a.x, ai.y - o.a.y, ai.z - o.a.z, gf * dx, gf * dy, gf * dz);
+            });
         const float4 gb = D.gb[s];
         sum += P.psi * dot3(ai.x, ai.y, ai.z, gb.x, gb.y, gb.z);
         const float Ap = dt2 * sum;
@@ -559,7 +628,7 @@ __global__ void k_sph_fields(const uint32_t* __restrict__ perm, const int* __res
 }
 
 __global__ void k_sph_export_nbrs(const uint32_t* __restrict__ perm, const int* __restrict__ fid,
-                                  int Nf, int N, int cap, const int* __restrict__ nbr,
+                                  int Nf, int N, int cap, const int4* __restrict__ nbr4,
                                   const int* __restrict__ ncnt, int* __restrict__ counts,
                                   int* __restrict__ idx) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -567,7 +636,12 @@ __global__ void k_sph_export_nbrs(const uint32_t* __restrict__ perm, const int* 
     const int id = fid[perm[s]];
     const int n = ncnt[s];
     counts[id] = n;
-    for (int k = 0; k < n && k < cap; ++k) idx[(size_t)k * N + id] = fid[perm[nbr[(size_t)k * Nf + s]]];
+    for (int k = 0; k < n && k < cap; ++k) {
+        const int4 q = nbr4[(size_t)(k >> 2) * Nf + s];
+        const int u = k & 3;
+        const int j = u == 0 ? q.x : (u == 1 ? q.y : (u == 2 ? q.z : q.w));
+        idx[(size_t)k * N + id] = fid[perm[j]];
+    }
 }
 
 __global__ void k_f32_zero(float* p, size_t n) {
@@ -604,8 +678,7 @@ using namespace nsph;
 // Kernel classes timed by nsph_solver_set_timing (names in nsph_kernel_name).
 enum KernelId {
     KT_BUILD = 0,  // cell clear + keys + radix sort + reorder/mark
-    KT_NEIGHBORS,
-    KT_DENSITY,
+    KT_NEIGHBORS,  // neighbour lists fused with the density pass
     KT_ADVECT,
     KT_SOURCE,
     KT_PACC,
@@ -615,7 +688,7 @@ enum KernelId {
     KT_PASSTHROUGH,
     KT_COUNT
 };
-static const char* kKernelNames[KT_COUNT] = {"build", "neighbors", "density", "advect", "source",
+static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source",
                                             "pressure_acc", "pressure_update", "reduce", "final",
                                             "passthrough"};
 
@@ -632,7 +705,7 @@ struct nsph_solver {
     size_t ev_used = 0;
     double t_ms[KT_COUNT] = {0};
     long t_count[KT_COUNT] = {0};
-    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0;
+    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0, cap4 = 0;
     int key_bits = 0;
     size_t total_cells = 0;
     double rho0 = 1000, omega = 0.5, tol = 1e-3;
@@ -685,7 +758,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.pr = bp<float>(S->pr);
     D.ap = bp<float4>(S->ap);
     D.pws = bp<float>(S->pws);
-    D.nbr = bp<int>(S->nbr);
+    D.nbr4 = bp<int4>(S->nbr);
     D.ncnt = bp<int>(S->ncnt);
     D.partial = bp<double>(S->partial);
     D.iters = bp<int>(S->iters);
@@ -777,6 +850,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     S->max_iters = prm->max_iters;
     S->warm = prm->warm_start;
     S->cap = prm->max_neighbors;
+    S->cap4 = (prm->max_neighbors + 3) / 4;
     S->v_max = m->v_max;
     S->d_mass = m->d_mass;
     S->d_inv_mass = m->d_inv_mass;
@@ -928,7 +1002,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
     NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
     NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
-    NSPH_TRY(buf_ensure(S->nbr, nf1 * (size_t)S->cap * 4));
+    NSPH_TRY(buf_ensure(S->nbr, nf1 * (size_t)S->cap4 * 16));
     NSPH_TRY(buf_ensure(S->ncnt, nf1 * 4));
     NSPH_TRY(buf_ensure(S->partial, ((size_t)S->C + 1) * 8));
     NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
@@ -1040,8 +1114,7 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
                            S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
                            Nf, D.xs, D.vs, D.pws, D.ftable);
         NSPH_TRY(toc(S, ep_build));
-        TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_neighbors, dim3(C), dim3(CHUNK), 0, st, D));
-        TIMED(KT_DENSITY, hipLaunchKernelGGL(k_sph_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
+        TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
         TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
         TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
         NSPH_HIP(hipGetLastError());
@@ -1124,7 +1197,7 @@ int nsph_solver_neighbors(nsph_solver* S, int* d_counts, int* d_idx) {
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_counts, N, 0);
     if (S->Nf > 0)
         hipLaunchKernelGGL(k_sph_export_nbrs, dim3((S->Nf + 255) / 256), dim3(256), 0, st, S->perm,
-                           bp<int>(S->fid), S->Nf, S->N, S->cap, bp<int>(S->nbr), bp<int>(S->ncnt),
+                           bp<int>(S->fid), S->Nf, S->N, S->cap, bp<int4>(S->nbr), bp<int>(S->ncnt),
                            d_counts, d_idx);
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;

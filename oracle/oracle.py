@@ -213,3 +213,28 @@ def sorted_order(pb: Problem, q_in):
     nf, nb = C.c_int(), C.c_int()
     L.orc_sorted_order(C.byref(s), _p(q_in), _p(ids), C.byref(nf), C.byref(nb))
     return ids[: nf.value].copy(), nf.value, nb.value
+
+
+def kernel_f64(H: float, r2: float) -> tuple[float, float]:
+    """Cubic spline of support H at squared distance r2: (W, (dW/dr)/r) in FP64."""
+    L = lib()
+    W, gf = C.c_double(), C.c_double()
+    L.orc_kernel_f64(C.c_double(H), C.c_double(r2), C.byref(W), C.byref(gf))
+    return W.value, gf.value
+
+
+def operator_f64(pb: Problem, q_in, qd_in, p_fluid, dt: float = 1e-4) -> dict:
+    """IISPH operator pieces in FP64 (fluid-list order): rho, aii, s, Ap = A p, ap = a^p(p)."""
+    L = lib()
+    s = pb._struct()
+    q_in = np.ascontiguousarray(q_in, dtype=np.float32)
+    qd_in = np.ascontiguousarray(qd_in, dtype=np.float32)
+    p = np.ascontiguousarray(p_fluid, dtype=np.float64)
+    nf = p.shape[0]
+    out = {k: np.zeros(nf, np.float64) for k in ("rho", "aii", "s", "Ap")}
+    out["ap"] = np.zeros((nf, 3), np.float64)
+    st = L.orc_operator_f64(C.byref(s), _p(q_in), _p(qd_in), C.c_float(dt), _p(p), _p(out["rho"]),
+                            _p(out["aii"]), _p(out["s"]), _p(out["Ap"]), _p(out["ap"]))
+    if st:
+        raise RuntimeError(f"orc_operator_f64 failed ({st})")
+    return out

@@ -341,6 +341,21 @@ int orc_step_f64(const orc_problem* pb, const float* q_in, const float* qd_in, f
                              rho_out, p_out, nbr_count_out, iters_out, err_out, nthreads);
 }
 
+/* Cubic spline of support H at r^2 (FP64): W and (dW/dr)/r (test hook). */
+void orc_kernel_f64(double H, double r2, double* W, double* gf) {
+    phys_t_f64 P;
+    const float g[3] = {0.f, 0.f, 0.f};
+    make_phys_f64(&P, H, 0.0, 1000.0, 0.0, g);
+    kern_wg_f64(&P, r2, W, gf);
+}
+
+/* IISPH operator pieces in FP64 (see orc_operator_impl). */
+int orc_operator_f64(const orc_problem* pb, const float* q_in, const float* qd_in, float dt,
+                     const double* p_in, double* rho, double* aii, double* s, double* Ap,
+                     double* ap) {
+    return orc_operator_impl_f64(pb, q_in, qd_in, dt, p_in, rho, aii, s, Ap, ap);
+}
+
 /* O(N^2) neighbour sets: for active i, all active j != i with FP32
  * (dx*dx + dy*dy) + dz*dz < H2, d = x_i - x_j, in ascending j.  Lists row-major i*cap + k.
  * Returns 3 when a list exceeds cap (counts hold the true sizes). */

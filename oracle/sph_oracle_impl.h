@@ -393,6 +393,116 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
     return status;
 }
 
+/* The pieces of one step before the Jacobi loop, and the IISPH operator applied to a given
+ * pressure field p_in (fluid-list order). Outputs in fluid-list order:
+ *   rho, a_ii, s (source), Ap = (A p_in), ap = a^p(p_in) (3 per particle).
+ * Test hook: lets tests check A against its diagonal, its linearity and the density sum. */
+static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const float* qd_in,
+                                 float dt, const double* p_in, double* rho_o, double* aii_o,
+                                 double* s_o, double* Ap_o, double* ap_o) {
+    const int W = pb->n_worlds;
+    orc_layout L;
+    int rc = orc_layout_build(pb, q_in, &L);
+    if (rc) return rc;
+    const int Nf = L.nf;
+    FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
+    for (int w = 0; w < W; ++w)
+        FN(make_phys)(&P[w], pb->support[w], pb->boundary_volume[w], pb->rest_density,
+                      pb->viscosity, pb->gravity + 3 * w);
+    const REAL rdt = (REAL)dt;
+    const REAL dt2 = (REAL)((double)dt * (double)dt);
+    REAL* xs = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)(Nf + 1));
+    REAL* rho = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
+    REAL* gb = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* pr = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
+    REAL* ap = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* va = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    for (int s = 0; s < Nf; ++s) {
+        const int id = L.fid[L.fperm[s]];
+        for (int a = 0; a < 3; ++a) xs[4 * s + a] = (REAL)q_in[3 * id + a];
+        xs[4 * s + 3] = (REAL)pb->mass[id];
+    }
+    const int cap = L.cap;
+    for (int s = 0; s < Nf; ++s) {
+        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
+        REAL sum = xs[4 * s + 3] * Pw->W0, cmp = 0, gfx = 0, gfy = 0, gfz = 0, S = 0;
+        for (int k = 0; k < L.ncnt[s]; ++k) {
+            const int j = L.nbr[(size_t)s * cap + k];
+            const REAL dx = xs[4 * s] - xs[4 * j], dy = xs[4 * s + 1] - xs[4 * j + 1], dz = xs[4 * s + 2] - xs[4 * j + 2];
+            REAL Wv, gf;
+            FN(kern_wg)(Pw, (dx * dx + dy * dy) + dz * dz, &Wv, &gf);
+            FN(neumaier)(&sum, &cmp, xs[4 * j + 3] * Wv);
+            gfx += xs[4 * j + 3] * gf * dx;
+            gfy += xs[4 * j + 3] * gf * dy;
+            gfz += xs[4 * j + 3] * gf * dz;
+            S += xs[4 * j + 3] * (gf * gf) * ((dx * dx + dy * dy) + dz * dz);
+        }
+        REAL gbx = 0, gby = 0, gbz = 0;
+        for (int k = L.bstart[s]; k < L.bstart[s + 1]; ++k) {
+            const int b = L.blist[k];
+            const REAL dx = xs[4 * s] - (REAL)L.bxs[3 * b], dy = xs[4 * s + 1] - (REAL)L.bxs[3 * b + 1],
+                       dz = xs[4 * s + 2] - (REAL)L.bxs[3 * b + 2];
+            REAL Wv, gf;
+            FN(kern_wg)(Pw, (dx * dx + dy * dy) + dz * dz, &Wv, &gf);
+            FN(neumaier)(&sum, &cmp, Pw->psi * Wv);
+            gbx += gf * dx;
+            gby += gf * dy;
+            gbz += gf * dz;
+        }
+        rho[s] = sum + cmp;
+        gb[3 * s] = gbx;
+        gb[3 * s + 1] = gby;
+        gb[3 * s + 2] = gbz;
+        const REAL Gx = gfx + Pw->psi * gbx, Gy = gfy + Pw->psi * gby, Gz = gfz + Pw->psi * gbz;
+        const int k = L.fperm[s];
+        rho_o[k] = (double)rho[s];
+        aii_o[k] = (double)(-(dt2 * ((FN(dot3)(Gx, Gy, Gz, Gx, Gy, Gz) + xs[4 * s + 3] * S) / (rho[s] * rho[s]))));
+        /* advected velocity without viscosity (the tests use nu = 0 here) */
+        const int id = L.fid[k];
+        for (int a = 0; a < 3; ++a) va[3 * s + a] = (REAL)qd_in[3 * id + a] + rdt * Pw->g[a];
+        pr[s] = (REAL)p_in[k] / (rho[s] * rho[s]);
+    }
+    for (int s = 0; s < Nf; ++s) {
+        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
+        REAL sum = 0, ax = 0, ay = 0, az = 0;
+        for (int k = 0; k < L.ncnt[s]; ++k) {
+            const int j = L.nbr[(size_t)s * cap + k];
+            const REAL dx = xs[4 * s] - xs[4 * j], dy = xs[4 * s + 1] - xs[4 * j + 1], dz = xs[4 * s + 2] - xs[4 * j + 2];
+            const REAL gf = FN(kern_gf)(Pw, (dx * dx + dy * dy) + dz * dz);
+            sum += xs[4 * j + 3] * FN(dot3)(va[3 * s] - va[3 * j], va[3 * s + 1] - va[3 * j + 1],
+                                            va[3 * s + 2] - va[3 * j + 2], gf * dx, gf * dy, gf * dz);
+            const REAL c = xs[4 * j + 3] * (pr[s] + pr[j]);
+            ax += c * (gf * dx);
+            ay += c * (gf * dy);
+            az += c * (gf * dz);
+        }
+        sum += Pw->psi * FN(dot3)(va[3 * s], va[3 * s + 1], va[3 * s + 2], gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]);
+        const REAL cb = Pw->psi * pr[s];
+        ap[3 * s] = -(ax + cb * gb[3 * s]);
+        ap[3 * s + 1] = -(ay + cb * gb[3 * s + 1]);
+        ap[3 * s + 2] = -(az + cb * gb[3 * s + 2]);
+        s_o[L.fperm[s]] = (double)((Pw->rho0 - rho[s]) - rdt * sum);
+    }
+    for (int s = 0; s < Nf; ++s) {
+        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
+        REAL sum = 0;
+        for (int k = 0; k < L.ncnt[s]; ++k) {
+            const int j = L.nbr[(size_t)s * cap + k];
+            const REAL dx = xs[4 * s] - xs[4 * j], dy = xs[4 * s + 1] - xs[4 * j + 1], dz = xs[4 * s + 2] - xs[4 * j + 2];
+            const REAL gf = FN(kern_gf)(Pw, (dx * dx + dy * dy) + dz * dz);
+            sum += xs[4 * j + 3] * FN(dot3)(ap[3 * s] - ap[3 * j], ap[3 * s + 1] - ap[3 * j + 1],
+                                            ap[3 * s + 2] - ap[3 * j + 2], gf * dx, gf * dy, gf * dz);
+        }
+        sum += Pw->psi * FN(dot3)(ap[3 * s], ap[3 * s + 1], ap[3 * s + 2], gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]);
+        const int k = L.fperm[s];
+        Ap_o[k] = (double)(dt2 * sum);
+        for (int a = 0; a < 3; ++a) ap_o[3 * k + a] = (double)ap[3 * s + a];
+    }
+    free(P); free(xs); free(rho); free(gb); free(pr); free(ap); free(va);
+    orc_layout_free(&L);
+    return 0;
+}
+
 #undef FN
 #undef CAT
 #undef CAT2
