@@ -90,10 +90,18 @@ def main():
 
     from newton_sph_dev_amd import SolverSPH, _native as N, scenes
 
-    cfg = {"dam_break_1m": scenes.DAM_BREAK_1M, "dam_break_32k": scenes.DAM_BREAK_32K}[args.workload]
-    print(f"[bench] building {cfg.name}", file=sys.stderr, flush=True)
-    model = scenes.build_dam_break(cfg).finalize("cuda:0")
-    solver = SolverSPH(model, cfg.support, warm_start=args.warm_start, check_every=args.check_every)
+    if args.workload.startswith("batched"):
+        cfg = {"batched_4096": scenes.BATCH_4096, "batched_1024": scenes.BATCH_1024}[args.workload]
+        print(f"[bench] building {cfg.name}", file=sys.stderr, flush=True)
+        builder, support = scenes.build_batched(cfg)
+        model = builder.finalize("cuda:0")
+        cfg.support = float(np.mean(support))
+    else:
+        cfg = {"dam_break_1m": scenes.DAM_BREAK_1M, "dam_break_32k": scenes.DAM_BREAK_32K}[args.workload]
+        print(f"[bench] building {cfg.name}", file=sys.stderr, flush=True)
+        model = scenes.build_dam_break(cfg).finalize("cuda:0")
+        support = cfg.support
+    solver = SolverSPH(model, support, warm_start=args.warm_start, check_every=args.check_every)
     s0, s1 = model.state(), model.state()
     q0 = model.particle_q.clone()
     qd0 = model.particle_qd.clone()
@@ -157,6 +165,7 @@ def main():
             "dt": dt,
             "tolerance": solver.tolerance,
             "warm_start": args.warm_start,
+            "worlds": solver.W,
             "mean_iterations": float(np.mean(iters)),
             "mean_fluid_neighbors": nbar,
         },

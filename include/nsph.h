@@ -143,6 +143,17 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* model, const nsph_s
 int nsph_solver_step(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
                      float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
                      int check_every, nsph_step_report* report);
+/* Capture one step from state_in to state_out (fixed pointers and dt) into HIP graphs:
+ * prologue + iter_slots Jacobi iterations, and the epilogue. Capture runs on a private
+ * stream; nothing is allocated on the device during capture or replay. *handle names the
+ * graph pair for nsph_solver_graph_step. */
+int nsph_solver_graph_capture(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
+                              float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
+                              int iter_slots, int* handle);
+/* Replay a captured step on the context stream. When iter_slots < max_iters the host reads
+ * the still-iterating world count once after the slots and, if needed, finishes the solve
+ * with eager iterations before the epilogue; results equal nsph_solver_step's. */
+int nsph_solver_graph_step(nsph_solver* s, int handle, nsph_step_report* report);
 /* Collect the report of the last step (synchronises the stream). */
 int nsph_solver_report(nsph_solver* s, nsph_step_report* report);
 /* Per-particle fields of the last step in particle order (any pointer may be NULL):

@@ -112,6 +112,8 @@ SIGNATURES: dict[str, list] = {
     "nsph_solver_create": [_vp, C.POINTER(ModelDesc), C.POINTER(SphParams), C.POINTER(_vp)],
     "nsph_solver_step": [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, C.POINTER(StepReport)],
     "nsph_solver_report": [_vp, C.POINTER(StepReport)],
+    "nsph_solver_graph_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, _pi],
+    "nsph_solver_graph_step": [_vp, _i, C.POINTER(StepReport)],
     "nsph_solver_fields": [_vp, _vp, _vp, _vp],
     "nsph_solver_world_stats": [_vp, _pi, _pd, _pi],
     "nsph_solver_neighbors": [_vp, _vp, _vp],

@@ -692,6 +692,17 @@ static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "adve
                                             "pressure_acc", "pressure_update", "reduce", "final",
                                             "passthrough"};
 
+struct StepGraph {
+    const float* q_in;
+    const float* qd_in;
+    float* q_out;
+    float* qd_out;
+    float* f_out;
+    float dt;
+    int slots;
+    hipGraphExec_t pro, epi;
+};
+
 struct EventPair {
     int kind;
     hipEvent_t a, b;
@@ -723,13 +734,15 @@ struct nsph_solver {
     DevBuf keys, vals;
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
-    int have_prev_keys = 0;
     RadixSort rs;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, pr, ap, pws, pprev, nbr, ncnt;
     DevBuf partial, iters, err, conv, act, flags;
     int* h_pinned = nullptr;  // [0] act value
     int stepped = 0;
     int last_iters_launched = 0;
+    int mlp_ready = 0;
+    hipStream_t cap_stream = nullptr;
+    std::vector<StepGraph> graphs;
 };
 
 static SphDev make_dev(nsph_solver* S) {
@@ -788,6 +801,13 @@ static void solver_free(nsph_solver* S) {
     radix_free(S->rs);
     if (S->h_pinned) (void)hipHostFree(S->h_pinned);
     S->h_pinned = nullptr;
+    for (StepGraph& g : S->graphs) {
+        if (g.pro) (void)hipGraphExecDestroy(g.pro);
+        if (g.epi) (void)hipGraphExecDestroy(g.epi);
+    }
+    S->graphs.clear();
+    if (S->cap_stream) (void)hipStreamDestroy(S->cap_stream);
+    S->cap_stream = nullptr;
     for (EventPair& e : S->ev_pool) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
@@ -988,6 +1008,10 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         NSPH_HIP(hipGetLastError());
     }
 
+    // the first step clears the cells named by S->skeys: point it at zero-filled keys (cell 0,
+    // already empty), after the boundary build has finished with the key buffer
+    hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->keys), (size_t)(nmax + 1), 0);
+    S->skeys = bp<uint32_t>(S->keys);
     const size_t nf1 = (size_t)Nf + 1;
     NSPH_TRY(buf_ensure(S->xs, nf1 * 16));
     NSPH_TRY(buf_ensure(S->vs, nf1 * 16));
@@ -1079,53 +1103,57 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* m, const nsph_sph_p
     return NSPH_OK;
 }
 
-int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
-                     float* qd_out, float* f_out, float dt, int check_every,
-                     nsph_step_report* report) {
-    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_step: NULL solver");
-    NSPH_CHECK(S->N == 0 || (q_in && qd_in && q_out && qd_out), NSPH_ERR_INVALID,
-               "nsph_solver_step: NULL state array");
-    NSPH_CHECK(dt > 0.f && std::isfinite(dt), NSPH_ERR_INVALID, "nsph_solver_step: dt must be positive");
-    NSPH_CHECK(S->N == 0 || (q_in != q_out && qd_in != qd_out), NSPH_ERR_INVALID,
-               "nsph_solver_step: state_in and state_out must not alias");
-    NSPH_CHECK(check_every >= 0, NSPH_ERR_INVALID, "nsph_solver_step: check_every must be >= 0");
-    NSPH_CHECK(S->warm != NSPH_WARM_MLP, NSPH_ERR_INVALID,
-               "nsph_solver_step: the MLP warm start needs nsph_solver_step_mlp");
-    hipStream_t st = S->ctx->stream;
+}  // extern "C"
+
+// ---- step pieces: prologue (build, lists, density, advection, source), Jacobi iterations,
+// epilogue (final pressure force, integration, non-fluid particles).  The eager step runs
+// them back to back; a captured step holds prologue + a fixed number of iteration slots in
+// one graph and the epilogue in another.
+
+static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
+                            float dt) {
     const int Nf = S->Nf, W = S->W, C = S->C;
     const float dt2 = (float)((double)dt * (double)dt);
     SphDev D = make_dev(S);
-
     hipLaunchKernelGGL(k_sph_step_init, dim3((W + 255) / 256), dim3(256), 0, st, D, W);
-    if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
-    if (Nf > 0) {
-        EventPair* ep_build;
-        NSPH_TRY(tic(S, KT_BUILD, &ep_build));
-        if (S->have_prev_keys)
-            hipLaunchKernelGGL(k_cells_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
-                               S->skeys, Nf, bp<int2>(S->ftable));
-        hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in,
-                           bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw), Nf,
-                           bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
-        NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
-                            st, &S->skeys, &S->perm));
-        S->have_prev_keys = 1;
-        hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
-                           S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
-                           Nf, D.xs, D.vs, D.pws, D.ftable);
-        NSPH_TRY(toc(S, ep_build));
-        TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
-        TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
-        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
-        NSPH_HIP(hipGetLastError());
-        const int red_blocks = (W * 64 + 255) / 256;
-        int launched = 0;
-        for (int it = 0; it < S->max_iters; ++it) {
+    if (Nf == 0) return NSPH_OK;
+    EventPair* ep_build;
+    NSPH_TRY(tic(S, KT_BUILD, &ep_build));
+    // the sorted keys of the previous build (zero-filled at creation) name the cells to clear
+    hipLaunchKernelGGL(k_cells_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
+                       S->skeys, Nf, bp<int2>(S->ftable));
+    hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in,
+                       bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw), Nf,
+                       bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+    NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
+                        st, &S->skeys, &S->perm));
+    hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
+                       S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
+                       Nf, D.xs, D.vs, D.pws, D.ftable);
+    NSPH_TRY(toc(S, ep_build));
+    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
+    TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
+    TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+// Iterations it0 .. it1-1; with check_every > 0 the host reads the number of still-iterating
+// worlds every check_every iterations (a stream sync) and stops launching when it is zero.
+static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0, int it1,
+                              int check_every, int* launched_to) {
+    const int W = S->W, C = S->C;
+    const float dt2 = (float)((double)dt * (double)dt);
+    SphDev D = make_dev(S);
+    const int red_blocks = (W * 64 + 255) / 256;
+    int launched = it0;
+    if (S->Nf > 0) {
+        for (int it = it0; it < it1; ++it) {
             TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C), dim3(CHUNK), 0, st, D));
             TIMED(KT_PUPDATE, hipLaunchKernelGGL(k_sph_pupdate, dim3(C), dim3(CHUNK), 0, st, D, dt2,
                                                  (float)S->omega));
-            TIMED(KT_REDUCE, hipLaunchKernelGGL(k_sph_reduce, dim3(red_blocks), dim3(256), 0, st, D, W,
-                                                it, S->tol, S->min_iters, S->max_iters));
+            TIMED(KT_REDUCE, hipLaunchKernelGGL(k_sph_reduce, dim3(red_blocks), dim3(256), 0, st, D,
+                                                W, it, S->tol, S->min_iters, S->max_iters));
             launched = it + 1;
             if (check_every > 0 && launched >= S->min_iters && (launched % check_every) == 0 &&
                 launched < S->max_iters) {
@@ -1135,11 +1163,20 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
                 if (S->h_pinned[0] == 0) break;
             }
         }
-        S->last_iters_launched = launched;
+    }
+    NSPH_HIP(hipGetLastError());
+    *launched_to = launched;
+    return NSPH_OK;
+}
+
+static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
+                            float* q_out, float* qd_out, float* f_out, float dt) {
+    const int W = S->W, C = S->C;
+    SphDev D = make_dev(S);
+    if (S->Nf > 0)
         TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C), dim3(CHUNK), 0, st, D, S->perm,
                                            bp<int>(S->fid), S->d_inv_mass, dt, S->v_max, q_out,
                                            qd_out, f_out, bp<float>(S->pprev)));
-    }
     if (S->No > 0) {
         TIMED(KT_PASSTHROUGH,
               hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
@@ -1148,6 +1185,124 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
                                  qd_out, f_out));
     }
     NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+static int check_step_args(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
+                           float* qd_out, float dt) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_step: NULL solver");
+    NSPH_CHECK(S->N == 0 || (q_in && qd_in && q_out && qd_out), NSPH_ERR_INVALID,
+               "nsph_solver_step: NULL state array");
+    NSPH_CHECK(dt > 0.f && std::isfinite(dt), NSPH_ERR_INVALID, "nsph_solver_step: dt must be positive");
+    NSPH_CHECK(S->N == 0 || (q_in != q_out && qd_in != qd_out), NSPH_ERR_INVALID,
+               "nsph_solver_step: state_in and state_out must not alias");
+    NSPH_CHECK(S->warm != NSPH_WARM_MLP || S->mlp_ready, NSPH_ERR_INVALID,
+               "nsph_solver_step: the MLP warm start needs weights (nsph_solver_set_mlp)");
+    return NSPH_OK;
+}
+
+extern "C" {
+
+int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
+                     float* qd_out, float* f_out, float dt, int check_every,
+                     nsph_step_report* report) {
+    NSPH_TRY(check_step_args(S, q_in, qd_in, q_out, qd_out, dt));
+    NSPH_CHECK(check_every >= 0, NSPH_ERR_INVALID, "nsph_solver_step: check_every must be >= 0");
+    hipStream_t st = S->ctx->stream;
+    if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
+    NSPH_TRY(enqueue_prologue(S, st, q_in, qd_in, dt));
+    int launched = 0;
+    NSPH_TRY(enqueue_iterations(S, st, dt, 0, S->max_iters, check_every, &launched));
+    S->last_iters_launched = launched;
+    NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt));
+    S->stepped = 1;
+    if (report) NSPH_TRY(collect_report(S, report));
+    return NSPH_OK;
+}
+
+int nsph_solver_graph_capture(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
+                              float* qd_out, float* f_out, float dt, int iter_slots, int* handle) {
+    NSPH_TRY(check_step_args(S, q_in, qd_in, q_out, qd_out, dt));
+    NSPH_CHECK(handle != nullptr, NSPH_ERR_INVALID, "nsph_solver_graph_capture: NULL handle");
+    NSPH_CHECK(iter_slots >= 1 && iter_slots <= S->max_iters, NSPH_ERR_INVALID,
+               "nsph_solver_graph_capture: iter_slots must be in [1, max_iters]");
+    if (!S->cap_stream) NSPH_HIP(hipStreamCreateWithFlags(&S->cap_stream, hipStreamNonBlocking));
+    NSPH_HIP(hipStreamSynchronize(S->ctx->stream));
+    const int timing = S->timing;
+    S->timing = 0;
+    StepGraph G{};
+    G.q_in = q_in;
+    G.qd_in = qd_in;
+    G.q_out = q_out;
+    G.qd_out = qd_out;
+    G.f_out = f_out;
+    G.dt = dt;
+    G.slots = iter_slots;
+    hipStream_t cs = S->cap_stream;
+    int rc = NSPH_OK;
+    for (int part = 0; part < 2 && rc == NSPH_OK; ++part) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = NSPH_ERR_HIP;
+            set_error("nsph_solver_graph_capture: hipStreamBeginCapture failed");
+            break;
+        }
+        int launched = 0;
+        if (part == 0) {
+            rc = enqueue_prologue(S, cs, q_in, qd_in, dt);
+            if (rc == NSPH_OK) rc = enqueue_iterations(S, cs, dt, 0, iter_slots, 0, &launched);
+        } else {
+            rc = enqueue_epilogue(S, cs, q_in, qd_in, q_out, qd_out, f_out, dt);
+        }
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc == NSPH_OK && e != hipSuccess) {
+            rc = NSPH_ERR_HIP;
+            set_error("nsph_solver_graph_capture: capture failed: %s", hipGetErrorString(e));
+        }
+        if (rc == NSPH_OK) {
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                rc = NSPH_ERR_HIP;
+                set_error("nsph_solver_graph_capture: instantiate failed: %s", hipGetErrorString(e));
+            } else if (part == 0) {
+                G.pro = ex;
+            } else {
+                G.epi = ex;
+            }
+        }
+        if (g) (void)hipGraphDestroy(g);
+    }
+    S->timing = timing;
+    if (rc != NSPH_OK) {
+        if (G.pro) (void)hipGraphExecDestroy(G.pro);
+        if (G.epi) (void)hipGraphExecDestroy(G.epi);
+        return rc;
+    }
+    S->graphs.push_back(G);
+    *handle = (int)S->graphs.size() - 1;
+    return NSPH_OK;
+}
+
+int nsph_solver_graph_step(nsph_solver* S, int handle, nsph_step_report* report) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_graph_step: NULL solver");
+    NSPH_CHECK(handle >= 0 && handle < (int)S->graphs.size(), NSPH_ERR_INVALID,
+               "nsph_solver_graph_step: unknown graph %d", handle);
+    const StepGraph& G = S->graphs[handle];
+    hipStream_t st = S->ctx->stream;
+    NSPH_HIP(hipGraphLaunch(G.pro, st));
+    int launched = G.slots;
+    if (G.slots < S->max_iters && S->Nf > 0) {
+        // the slots may not have been enough: read the still-iterating count once and finish
+        // eagerly (same kernels, same order, so results equal the eager step's)
+        NSPH_HIP(hipMemcpyAsync(S->h_pinned, bp<int>(S->act) + ((G.slots - 1) & 1), 4,
+                                hipMemcpyDeviceToHost, st));
+        NSPH_HIP(hipStreamSynchronize(st));
+        if (S->h_pinned[0] != 0)
+            NSPH_TRY(enqueue_iterations(S, st, G.dt, G.slots, S->max_iters, 1, &launched));
+    }
+    S->last_iters_launched = launched;
+    NSPH_HIP(hipGraphLaunch(G.epi, st));
     S->stepped = 1;
     if (report) NSPH_TRY(collect_report(S, report));
     return NSPH_OK;

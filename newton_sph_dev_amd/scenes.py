@@ -106,6 +106,10 @@ BATCH_4096 = BatchedWorlds("batched_4096", 4096, (256, 8192), True)
 BATCH_1024 = BatchedWorlds("batched_1024", 1024, (4096, 65536), False)
 
 
+def small_batch(n_worlds: int = 12, seed: int = 7) -> BatchedWorlds:
+    return BatchedWorlds(f"batched_{n_worlds}", n_worlds, (256, 2048), True, seed=seed)
+
+
 def build_batched(cfg: BatchedWorlds, rest_density: float = REST_DENSITY):
     """Independent worlds: per world a random water block (extents U(0.2, 0.6) m) with
     about N_w particles (N_w log-uniform or uniform in count_range), in an open box twice
@@ -151,3 +155,22 @@ def build_batched(cfg: BatchedWorlds, rest_density: float = REST_DENSITY):
         b.end_world()
         supports.append(2.0 * s)
     return b, np.asarray(supports)
+
+
+def extract_world(model, w: int) -> ModelBuilder:
+    """A one-world builder holding world w's particles of a finalized model (positions as
+    stored, float32 -> float64 exactly), with that world's gravity."""
+    q = model.particle_q.detach().cpu().numpy()
+    qd = model.particle_qd.detach().cpu().numpy()
+    mass = model.particle_mass.detach().cpu().numpy()
+    flags = model.particle_flags.detach().cpu().numpy()
+    radius = model.particle_radius.detach().cpu().numpy()
+    world = model.particle_world.detach().cpu().numpy()
+    g = model.gravity.detach().cpu().numpy().reshape(-1, 3)[w]
+    sel = world == w
+    b = ModelBuilder()
+    b.begin_world(gravity=tuple(float(x) for x in g))
+    b.add_particles(q[sel].astype(np.float64), qd[sel].astype(np.float64), mass[sel].astype(np.float64),
+                    radius[sel].astype(np.float64), flags[sel])
+    b.end_world()
+    return b

@@ -255,6 +255,31 @@ class SolverSPH(SolverBase):
             if rep.status == N.NSPH_ERR_NONFINITE:
                 raise N.NsphError(rep.status, "non-finite position or velocity after the step")
 
+    # ---- captured steps (HIP graphs) ------------------------------------------------------------
+    def capture(self, state_in: State, state_out: State, dt: float, iter_slots: int | None = None) -> int:
+        """Capture one step state_in -> state_out into HIP graphs and return its handle.
+
+        The graph holds the prologue and `iter_slots` Jacobi iterations (default: max_iters,
+        fully asynchronous) plus the epilogue; with fewer slots the replay reads the
+        convergence count once and finishes eagerly if needed. The tensors must stay alive
+        and keep their storage; dt is fixed at capture."""
+        N.set_stream(self._ctx, torch.cuda.current_stream(self.model.device).cuda_stream)
+        slots = self.max_iters if iter_slots is None else int(iter_slots)
+        h = C.c_int()
+        N.call("nsph_solver_graph_capture", self._handle, N.ptr(state_in.particle_q),
+               N.ptr(state_in.particle_qd), N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd),
+               N.ptr(state_out.particle_f), float(dt), slots, C.byref(h))
+        self._graph_refs = getattr(self, "_graph_refs", []) + [(state_in, state_out)]
+        return h.value
+
+    def replay(self, handle: int) -> None:
+        """Run a captured step on the current stream (report as for step())."""
+        N.set_stream(self._ctx, torch.cuda.current_stream(self.model.device).cuda_stream)
+        rep = N.StepReport() if self.report else None
+        N.call("nsph_solver_graph_step", self._handle, int(handle), C.byref(rep) if rep is not None else None)
+        if rep is not None:
+            self.last_report = rep.as_dict()
+
     def collect_report(self) -> dict:
         rep = N.StepReport()
         N.call("nsph_solver_report", self._handle, C.byref(rep))
