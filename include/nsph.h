@@ -46,6 +46,7 @@ enum nsph_status {
 typedef struct nsph_ctx nsph_ctx;
 typedef struct nsph_grid nsph_grid;
 typedef struct nsph_solver nsph_solver;
+typedef struct nsph_mlp nsph_mlp;
 
 /* ---- context --------------------------------------------------------------------- */
 
@@ -181,6 +182,36 @@ const char* nsph_kernel_name(int k);
  * applied when check_every > 1 or 0; the extra ones exit at once). */
 int nsph_solver_iterations_launched(const nsph_solver* s, int* n);
 int nsph_solver_destroy(nsph_solver* s);
+
+/* ---- learned warm start: per-particle MLP 6 -> 64 -> 64 -> 1 (BF16 MFMA, FP32 accumulate) ----
+ * No counterpart in the reference (which has no pressure solve).  The closest analog is the
+ * ONNX MLP of ControllerNeuralMLP (newton/_src/actuators/controllers/controller_neural_mlp.py:131,
+ * run by newton/_src/utils/onnx_runtime.py:69-87).
+ * Parameters, flat FP32 (nsph_mlp_num_params() = 4673): W1[6][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3.  Feature rows are 8 floats (6 features, 2 ignored). */
+int nsph_mlp_num_params(void);
+int nsph_mlp_create(nsph_ctx* ctx, nsph_mlp** out);
+int nsph_mlp_destroy(nsph_mlp* m);
+/* Host parameter copy in/out (set also resets the Adam moments). */
+int nsph_mlp_set_params(nsph_mlp* m, const float* h_params);
+int nsph_mlp_get_params(nsph_mlp* m, float* h_params);
+/* d_y[i] = MLP(d_x[8 i .. 8 i + 5]) for i < n. */
+int nsph_mlp_forward(nsph_mlp* m, const float* d_x, int n, float* d_y);
+/* Mean squared error of MLP(x) against d_t and its gradient (kept for nsph_mlp_adam; also
+ * copied to d_grad[0 .. 4672] when non-NULL, d_grad[4673] = sum of squared errors).
+ * *h_loss (optional, synchronises) receives the mean squared error. Deterministic. */
+int nsph_mlp_grad(nsph_mlp* m, const float* d_x, const float* d_t, int n, float* h_loss,
+                  float* d_grad);
+/* One Adam step (FP32 master weights) with the last gradient; refreshes the BF16 copies. */
+int nsph_mlp_adam(nsph_mlp* m, float lr, float beta1, float beta2, float eps);
+/* One 16x16x32 BF16 MFMA product C = A B (A 16x32, B 32x16 row-major host arrays, rounded to
+ * BF16) through the operand layout the MLP kernels use; a self-check of that layout. */
+int nsph_mfma_selftest(nsph_ctx* ctx, const float* h_a, const float* h_b, float* h_c);
+/* Attach an MLP to a solver (NULL detaches); warm_start NSPH_WARM_MLP then predicts p0 from
+ * the particle features before the Jacobi iterations, which still run to the tolerance. */
+int nsph_solver_set_mlp(nsph_solver* s, nsph_mlp* m);
+/* Features (n_fluid x 8) and target p / p_unit (n_fluid) of the last step, cell-sorted order. */
+int nsph_solver_mlp_features(nsph_solver* s, float* d_x, float* d_target);
 
 #ifdef __cplusplus
 }

@@ -125,6 +125,17 @@ SIGNATURES: dict[str, list] = {
     "nsph_kernel_count": [],
     "nsph_solver_iterations_launched": [_vp, _pi],
     "nsph_solver_destroy": [_vp],
+    "nsph_mlp_num_params": [],
+    "nsph_mlp_create": [_vp, C.POINTER(_vp)],
+    "nsph_mlp_destroy": [_vp],
+    "nsph_mlp_set_params": [_vp, C.POINTER(C.c_float)],
+    "nsph_mlp_get_params": [_vp, C.POINTER(C.c_float)],
+    "nsph_mlp_forward": [_vp, _vp, _i, _vp],
+    "nsph_mlp_grad": [_vp, _vp, _vp, _i, C.POINTER(C.c_float), _vp],
+    "nsph_mlp_adam": [_vp, _f, _f, _f, _f],
+    "nsph_mfma_selftest": [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)],
+    "nsph_solver_set_mlp": [_vp, _vp],
+    "nsph_solver_mlp_features": [_vp, _vp, _vp],
 }
 
 _lib: C.CDLL | None = None

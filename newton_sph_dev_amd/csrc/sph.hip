@@ -33,6 +33,10 @@
 #include <vector>
 
 #include "common.h"
+#include "mlp_dev.h"
+
+// device view of an MLP handle (defined in mlp.hip)
+nsph::MlpDev nsph_mlp_device_view(const nsph_mlp* m);
 
 namespace nsph {
 
@@ -114,6 +118,7 @@ struct SphDev {
     float* pws;  // warm-start input in sorted order
     int4* nbr4;
     int* ncnt;
+    float2* aux;  // (boundary neighbours, distance to the nearest boundary particle / H, 1 if none)
     // iteration control
     double* partial;
     int* iters;
@@ -203,7 +208,9 @@ __device__ inline int2 row_range(const int2* __restrict__ table, int row, int x0
 
 // Candidates j of the rows of the (clamped) query box around (qx,qy,qz), in the order z, y,
 // then ascending sorted index; calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).
-// Positions are fetched four at a time.
+// All row ranges are looked up first (their table loads are independent), then positions
+// are fetched RV_BATCH at a time.
+#define RV_BATCH 8
 template <class F>
 __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table,
                                   const float4* __restrict__ xs, float qx, float qy, float qz,
@@ -213,16 +220,55 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
     axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
     axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
+    if (nx <= 3 && ny <= 3 && nz <= 3) {
+        // common case: up to 3 x 3 rows of up to 3 cells, table loads issued together
+        int2 rg[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int iz = k / 3, iy = k % 3;
+            rg[k] = make_int2(0x7fffffff, 0);
+            if (iz < nz && iy < ny) {
+                const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
+                int2 c[3];
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix)
+                    if (ix < nx && c[ix].x < c[ix].y) {
+                        rg[k].x = min(rg[k].x, c[ix].x);
+                        rg[k].y = max(rg[k].y, c[ix].y);
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int hi = rg[k].y;
+            for (int j = rg[k].x; j < hi; j += RV_BATCH) {
+                float4 p[RV_BATCH];
+#pragma unroll
+                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
+#pragma unroll
+                for (int u = 0; u < RV_BATCH; ++u) {
+                    if (j + u < hi) {
+                        const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
+                        const float d2 = (dx * dx + dy * dy) + dz * dz;
+                        if (d2 < r2max) f(j + u, dx, dy, dz, d2);
+                    }
+                }
+            }
+        }
+        return;
+    }
     for (int iz = 0; iz < nz; ++iz) {
         for (int iy = 0; iy < ny; ++iy) {
             const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0];
             const int2 rg = row_range(table, row, x0, nx);
-            for (int j = rg.x; j < rg.y; j += 4) {
-                float4 p[4];
+            for (int j = rg.x; j < rg.y; j += RV_BATCH) {
+                float4 p[RV_BATCH];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) p[u] = xs[min(j + u, rg.y - 1)];
+                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, rg.y - 1)];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < RV_BATCH; ++u) {
                     if (j + u < rg.y) {
                         const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
                         const float d2 = (dx * dx + dy * dy) + dz * dz;
@@ -234,12 +280,52 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     }
 }
 
-// Neighbour lists fused with the density pass.
-//   list:  j != i, |x_i - x_j|^2 < H^2 (FP32 predicate), candidate order
-//   rho_i = m_i W(0) + sum_j m_j W_ij + psi sum_b W_ib     (Neumaier, list then boundary order)
-//   Gf = sum_j m_j gradW_ij,  S = sum_j m_j |gradW_ij|^2,  Gb = sum_b gradW_ib
-//   G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
-// a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.
+// Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
+// (B = 4 or 8 entries, one or two quads, at a time) and use(data, j) accumulates in slot
+// order.  Missing entries of the last batch repeat the first one and are not used.
+template <class T, int B, class Load, class Use>
+__device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) {
+    static_assert(B == 4 || B == 8, "batch of 4 or 8");
+    const int n = D.ncnt[s];
+    const int4* __restrict__ L = D.nbr4 + s;
+    const size_t stride = (size_t)D.Nf;
+    for (int k = 0; k < n; k += B) {
+        const int m = n - k;
+        const int4 q0 = L[(size_t)(k >> 2) * stride];
+        const int j0 = q0.x, j1 = m > 1 ? q0.y : j0, j2 = m > 2 ? q0.z : j0, j3 = m > 3 ? q0.w : j0;
+        if (B == 4) {
+            const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
+            use(a0, j0);
+            if (m > 1) use(a1, j1);
+            if (m > 2) use(a2, j2);
+            if (m > 3) use(a3, j3);
+        } else {
+            const int4 q1 = L[(size_t)((k >> 2) + (m > 4 ? 1 : 0)) * stride];
+            const int j4 = m > 4 ? q1.x : j0, j5 = m > 5 ? q1.y : j0, j6 = m > 6 ? q1.z : j0,
+                      j7 = m > 7 ? q1.w : j0;
+            const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
+            const T a4 = load(j4), a5 = load(j5), a6 = load(j6), a7 = load(j7);
+            use(a0, j0);
+            if (m > 1) use(a1, j1);
+            if (m > 2) use(a2, j2);
+            if (m > 3) use(a3, j3);
+            if (m > 4) use(a4, j4);
+            if (m > 5) use(a5, j5);
+            if (m > 6) use(a6, j6);
+            if (m > 7) use(a7, j7);
+        }
+    }
+}
+
+// Neighbour lists fused with the density pass, in two phases so that the expensive
+// per-neighbour work is not paid for every candidate of a divergent wave:
+//   phase 1  list:  j != i, |x_i - x_j|^2 < H^2 (FP32 predicate), candidate order
+//   phase 2  rho_i = m_i W(0) + sum_j m_j W_ij + psi sum_b W_ib   (Neumaier, list then
+//                    boundary order)
+//            Gf = sum_j m_j gradW_ij,  S = sum_j m_j |gradW_ij|^2,  Gb = sum_b gradW_ib
+//            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
+// a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
+// vs[s].w so that the viscosity pass fetches velocity and density together.
 __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot(D, &w, &s)) return;
@@ -247,42 +333,36 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const int Nf = D.Nf, cap = D.cap;
-    int4* __restrict__ out = D.nbr4 + s;
+    int* __restrict__ out = reinterpret_cast<int*>(D.nbr4) + 4 * (size_t)s;
     int c = 0;
-    int4 quad = make_int4(0, 0, 0, 0);
-    float sum = xi.w * P.W0, cmp = 0.0f;
-    float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
     rows_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
-               [&](int j, float dx, float dy, float dz, float r2) {
+               [&](int j, float, float, float, float) {
                    if (j == s) return;
-                   if (c < cap) {
-                       const int u = c & 3;
-                       if (u == 0) quad.x = j;
-                       else if (u == 1) quad.y = j;
-                       else if (u == 2) quad.z = j;
-                       else {
-                           quad.w = j;
-                           out[(size_t)(c >> 2) * Nf] = quad;
-                       }
-                   }
+                   if (c < cap) out[(size_t)(c >> 2) * 4 * Nf + (c & 3)] = j;
                    ++c;
-                   const float mj = D.xs[j].w;
-                   float W, gf;
-                   kern_wg(P, r2, &W, &gf);
-                   neumaier(sum, cmp, mj * W);
-                   const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
-                   gfx += mj * gx;
-                   gfy += mj * gy;
-                   gfz += mj * gz;
-                   S += mj * dot3(gx, gy, gz, gx, gy, gz);
                });
-    const int cl = c < cap ? c : cap;
-    if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
-    D.ncnt[s] = cl;
+    D.ncnt[s] = c < cap ? c : cap;
     if (c > cap) atomicOr(&D.flags[0], 1);
     atomicMax(&D.flags[1], c);
 
-    float gbx = 0.f, gby = 0.f, gbz = 0.f;
+    float sum = xi.w * P.W0, cmp = 0.0f;
+    float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
+    for_each_nbr<float4, 8>(
+        D, s, [&](int j) { return D.xs[j]; },
+        [&](const float4& xj, int) {
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            float W, gf;
+            kern_wg(P, r2, &W, &gf);
+            neumaier(sum, cmp, xj.w * W);
+            const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+            gfx += xj.w * gx;
+            gfy += xj.w * gy;
+            gfz += xj.w * gz;
+            S += xj.w * dot3(gx, gy, gz, gx, gy, gz);
+        });
+
+    float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
     int nb = 0;
     rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                [&](int, float dx, float dy, float dz, float r2) {
@@ -292,38 +372,21 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
                    gbx += gf * dx;
                    gby += gf * dy;
                    gbz += gf * dz;
+                   mr2 = fminf(mr2, r2);
                    ++nb;
                });
     const float rho = sum + cmp;
     const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
     const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + xi.w * S;
     D.rho[s] = rho;
+    reinterpret_cast<float*>(D.vs + s)[3] = rho;
+    D.aux[s] = make_float2((float)nb, nb > 0 ? sqrtf(mr2) * P.invH : 1.0f);
     D.aii[s] = -(dt2 * (num / (rho * rho)));
     D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
 }
 
-// Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
-// (four entries at a time) and use(data, j) accumulates.
-template <class T, class Load, class Use>
-__device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) {
-    const int n = D.ncnt[s];
-    const int4* __restrict__ L = D.nbr4 + s;
-    const size_t stride = (size_t)D.Nf;
-    for (int k = 0; k < n; k += 4) {
-        const int4 q = L[(size_t)(k >> 2) * stride];
-        const int m = n - k;
-        const int j0 = q.x, j1 = m > 1 ? q.y : q.x, j2 = m > 2 ? q.z : q.x, j3 = m > 3 ? q.w : q.x;
-        const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
-        use(a0, j0);
-        if (m > 1) use(a1, j1);
-        if (m > 2) use(a2, j2);
-        if (m > 3) use(a3, j3);
-    }
-}
-
 struct XV {
-    float4 x, v;
-    float r;
+    float4 x, v;  // v.w = rho
 };
 struct XS {
     float4 x;
@@ -343,14 +406,14 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XV>(
-        D, s, [&](int j) { return XV{D.xs[j], D.vs[j], D.rho[j]}; },
+    for_each_nbr<XV, 4>(
+        D, s, [&](int j) { return XV{D.xs[j], D.vs[j]}; },
         [&](const XV& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
             const float gf = kern_gf(P, r2);
             const float vx = dot3(vi.x - o.v.x, vi.y - o.v.y, vi.z - o.v.z, dx, dy, dz);
-            const float c = (o.x.w / o.r) * (vx / (r2 + P.eps2));
+            const float c = (o.x.w / o.v.w) * (vx / (r2 + P.eps2));
             ax += c * (gf * dx);
             ay += c * (gf * dy);
             az += c * (gf * dz);
@@ -371,7 +434,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
     const float4 xi = D.xs[s];
     const float4 vi = D.vadv[s];
     float sum = 0.f;
-    for_each_nbr<XA>(
+    for_each_nbr<XA, 4>(
         D, s, [&](int j) { return XA{D.xs[j], D.vadv[j]}; },
         [&](const XA& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
@@ -395,7 +458,7 @@ __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int s, flo
                                  float* ox, float* oy, float* oz) {
     const float pri = D.pr[s];
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XS>(
+    for_each_nbr<XS, 8>(
         D, s, [&](int j) { return XS{D.xs[j], D.pr[j]}; },
         [&](const XS& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
@@ -440,7 +503,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
         const float4 xi = D.xs[s];
         const float4 ai = D.ap[s];
         float sum = 0.f;
-        for_each_nbr<XA>(
+        for_each_nbr<XA, 4>(
             D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
             [&](const XA& o, int) {
                 const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
@@ -543,6 +606,87 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
     if (!(isfinite(x1) && isfinite(y1) && isfinite(z1) && isfinite(v1x) && isfinite(v1y) &&
           isfinite(v1z)))
         atomicOr(&D.flags[2], 1);
+}
+
+// ---- learned warm start -----------------------------------------------------------------
+//
+// Features of sorted fluid particle s (all from the prologue's buffers):
+//   f0 = 100 (rho - rho0) / rho0                  density error, %
+//   f1 = 100 ((rho0 - rho) - s_i) / rho0          = 100 dt (sum_j m_j v_ij . gradW_ij) / rho0,
+//                                                   density change from the velocity divergence, %
+//   f2 = (fluid + boundary neighbours) / 32
+//   f3 = p_prev / p_unit                          previous pressure
+//   f4 = 100 |v_adv| dt / H                       displacement per step, % of H
+//   f5 = distance to the nearest boundary particle / H (1 if none within H)
+// p_unit = 0.01 rho0 / (-a_ii) is the pressure that one Jacobi step assigns to a 1 % density
+// error; the network predicts p / p_unit and the warm start is p0 = max(y, 0) p_unit.
+__device__ inline float mlp_features(const SphDev& D, const PhysWorld& P, int s, float dt,
+                                     float* f) {
+    const float rho = D.rho[s], src = D.src[s], a = D.aii[s];
+    const float2 ax = D.aux[s];
+    const float4 v = D.vadv[s];
+    const float punit = a < 0.0f ? (P.rho0 * 0.01f) / (-a) : 0.0f;
+    f[0] = (rho - P.rho0) / P.rho0 * 100.0f;
+    f[1] = ((P.rho0 - rho) - src) / P.rho0 * 100.0f;
+    f[2] = ((float)D.ncnt[s] + ax.x) * (1.0f / 32.0f);
+    f[3] = punit > 0.0f ? D.pws[s] / punit : 0.0f;
+    f[4] = sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)) * dt * P.invH * 100.0f;
+    f[5] = ax.y;
+    f[6] = 0.0f;
+    f[7] = 0.0f;
+    return punit;
+}
+
+// Features and the training target (converged pressure / p_unit) of the last step, sorted order.
+__global__ __launch_bounds__(CHUNK) void k_sph_mlp_features(SphDev D, float dt, float* __restrict__ X,
+                                                            float* __restrict__ T) {
+    int w, s;
+    if (!chunk_slot(D, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    float f[MLP_XPAD];
+    const float punit = mlp_features(D, P, s, dt, f);
+#pragma unroll
+    for (int j = 0; j < MLP_XPAD; ++j) X[(size_t)s * MLP_XPAD + j] = f[j];
+    T[s] = punit > 0.0f ? D.p[s] / punit : 0.0f;
+}
+
+// Warm start: p0 = max(MLP(features), 0) p_unit, written to p and pr before the iterations.
+__global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, float dt) {
+    __shared__ __align__(16) float xs[CHUNK * MLP_XPAD];
+    __shared__ __align__(16) __bf16 h1s[CHUNK / 64][16 * MLP_H];
+    __shared__ float ys[CHUNK];
+    int w, s;
+    const bool ok = chunk_slot(D, &w, &s);
+    const int t = threadIdx.x;
+    const PhysWorld P = D.phys[w];
+    float f[MLP_XPAD];
+    float punit = 0.0f;
+    if (ok) {
+        punit = mlp_features(D, P, s, dt, f);
+    } else {
+#pragma unroll
+        for (int j = 0; j < MLP_XPAD; ++j) f[j] = 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < MLP_XPAD; ++j) xs[t * MLP_XPAD + j] = f[j];
+    __syncthreads();
+    const int wave = t >> 6, l = t & 63, g = l >> 4, c = l & 15;
+    MlpRegs R;
+    mlp_load_regs(M, R);
+    for (int h = 0; h < 4; ++h) {
+        float y[4];
+        f32x4 c2[4];
+        mlp_tile_forward(R, xs + (wave * 64 + 16 * h) * MLP_XPAD, h1s[wave], y, c2);
+        if (c < 4) ys[wave * 64 + 16 * h + 4 * g + c] = c == 0 ? y[0] : (c == 1 ? y[1] : (c == 2 ? y[2] : y[3]));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    if (ok) {
+        const float p0 = fmaxf(ys[t], 0.0f) * punit;
+        const float rho = D.rho[s];
+        D.p[s] = p0;
+        D.pr[s] = p0 / (rho * rho);
+    }
 }
 
 // Non-fluid particles, reference integrator with zero force (solver.py:24-46).
@@ -681,6 +825,7 @@ enum KernelId {
     KT_NEIGHBORS,  // neighbour lists fused with the density pass
     KT_ADVECT,
     KT_SOURCE,
+    KT_MLP,
     KT_PACC,
     KT_PUPDATE,
     KT_REDUCE,
@@ -688,7 +833,7 @@ enum KernelId {
     KT_PASSTHROUGH,
     KT_COUNT
 };
-static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source",
+static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source", "mlp_warm_start",
                                             "pressure_acc", "pressure_update", "reduce", "final",
                                             "passthrough"};
 
@@ -735,12 +880,14 @@ struct nsph_solver {
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
-    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, pr, ap, pws, pprev, nbr, ncnt;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, pr, ap, pws, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags;
     int* h_pinned = nullptr;  // [0] act value
     int stepped = 0;
     int last_iters_launched = 0;
     int mlp_ready = 0;
+    nsph_mlp* mlp = nullptr;
+    float last_dt = 0.f;
     hipStream_t cap_stream = nullptr;
     std::vector<StepGraph> graphs;
 };
@@ -773,6 +920,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.pws = bp<float>(S->pws);
     D.nbr4 = bp<int4>(S->nbr);
     D.ncnt = bp<int>(S->ncnt);
+    D.aux = bp<float2>(S->aux);
     D.partial = bp<double>(S->partial);
     D.iters = bp<int>(S->iters);
     D.err = bp<double>(S->err);
@@ -795,7 +943,7 @@ static void solver_free(nsph_solver* S) {
                       &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->pr, &S->ap, &S->pws, &S->pprev,
-                      &S->nbr, &S->ncnt, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
+                      &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
@@ -1028,6 +1176,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
     NSPH_TRY(buf_ensure(S->nbr, nf1 * (size_t)S->cap4 * 16));
     NSPH_TRY(buf_ensure(S->ncnt, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->aux, nf1 * 8));
     NSPH_TRY(buf_ensure(S->partial, ((size_t)S->C + 1) * 8));
     NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->err, ((size_t)W + 1) * 8));
@@ -1134,7 +1283,11 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
     TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
+    if (S->warm == NSPH_WARM_MLP)
+        TIMED(KT_MLP, hipLaunchKernelGGL(k_sph_mlp_warm, dim3(C), dim3(CHUNK), 0, st, D,
+                                         nsph_mlp_device_view(S->mlp), dt));
     NSPH_HIP(hipGetLastError());
+    S->last_dt = dt;
     return NSPH_OK;
 }
 
@@ -1379,6 +1532,25 @@ int nsph_solver_num_fluid(const nsph_solver* S, int* n_fluid, int* n_boundary) {
     NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_num_fluid: NULL solver");
     if (n_fluid) *n_fluid = S->Nf;
     if (n_boundary) *n_boundary = S->Nb;
+    return NSPH_OK;
+}
+
+int nsph_solver_set_mlp(nsph_solver* S, nsph_mlp* mlp) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_mlp: NULL solver");
+    S->mlp = mlp;
+    S->mlp_ready = mlp != nullptr;
+    if (!mlp && S->warm == NSPH_WARM_MLP) S->warm = NSPH_WARM_ZERO;
+    return NSPH_OK;
+}
+
+int nsph_solver_mlp_features(nsph_solver* S, float* d_x, float* d_target) {
+    NSPH_CHECK(S && d_x && d_target, NSPH_ERR_INVALID, "nsph_solver_mlp_features: NULL argument");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_mlp_features: no step has run");
+    if (S->Nf > 0) {
+        hipLaunchKernelGGL(k_sph_mlp_features, dim3(S->C), dim3(CHUNK), 0, S->ctx->stream, make_dev(S),
+                           S->last_dt, d_x, d_target);
+        NSPH_HIP(hipGetLastError());
+    }
     return NSPH_OK;
 }
 

@@ -219,6 +219,7 @@ class SolverSPH(SolverBase):
         pr.max_iters = self.max_iters
         pr.max_neighbors = self.max_neighbors
         pr.warm_start = _WARM[self.warm_start] if self.warm_start != "mlp" else N.WARM_ZERO
+        self._mlp = None
         h = C.c_void_p()
         N.call("nsph_solver_create", self._ctx, C.byref(md), C.byref(pr), C.byref(h))
         self._handle = h
@@ -226,6 +227,21 @@ class SolverSPH(SolverBase):
         N.call("nsph_solver_num_fluid", h, C.byref(nf), C.byref(nb))
         self.fluid_count = nf.value
         self.boundary_count = nb.value
+
+    def set_mlp(self, mlp) -> None:
+        """Attach a learned warm start (newton_sph_dev_amd.learned.WarmStartMLP) and switch
+        warm_start to "mlp"; None detaches it and falls back to "zero"."""
+        N.call("nsph_solver_set_mlp", self._handle, None if mlp is None else mlp._handle)
+        self._mlp = mlp
+        self.set_warm_start("mlp" if mlp is not None else "zero")
+
+    def set_warm_start(self, mode: str) -> None:
+        if mode not in _WARM:
+            raise ValueError(f"warm_start must be one of {sorted(_WARM)}")
+        if mode == "mlp" and self._mlp is None:
+            raise ValueError("warm_start='mlp' needs set_mlp() first")
+        N.call("nsph_solver_set_warm_start", self._handle, _WARM[mode])
+        self.warm_start = mode
 
     def notify_model_changed(self, flags: int) -> None:
         """MODEL_PROPERTIES: re-create (re-classify particles, re-hash the static boundary)."""

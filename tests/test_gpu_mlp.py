@@ -1,0 +1,149 @@
+"""The BF16 MFMA MLP kernels against the FP64 restatement (oracle/mlp_oracle.py), and the
+learned warm start inside the solver.
+
+Bars: MFMA operand layout exact on a BF16-representable product; forward within 1e-5
+(relative to max |y|) of the BF16-emulating FP64 oracle; gradient within 2e-3 norm-wise
+(BF16 roundings of intermediates can flip by one unit where FP32 and FP64 sums straddle a
+rounding boundary); bitwise repeatable gradients; an all-zero network reproduces the zero
+warm start bit for bit; the MLP warm start meets the same tolerance as the plain solver.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from newton_sph_dev_amd import SolverSPH, _native as N, scenes
+from newton_sph_dev_amd.learned import N_PARAM, WarmStartMLP, features
+from oracle import mlp_oracle as MO
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_mfma_operand_layout():
+    _print("mfma 16x16x32 bf16 layout self-test")
+    rng = np.random.default_rng(0)
+    A = MO.bf16(rng.standard_normal((16, 32))).astype(np.float32)
+    B = MO.bf16(rng.standard_normal((32, 16))).astype(np.float32)
+    Cm = np.zeros((16, 16), np.float32)
+    ctx = N.context(0)
+    N.call("nsph_mfma_selftest", ctx, A.ctypes.data_as(C.POINTER(C.c_float)),
+           B.ctypes.data_as(C.POINTER(C.c_float)), Cm.ctypes.data_as(C.POINTER(C.c_float)))
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.allclose(Cm, ref, rtol=1e-5, atol=1e-5)
+
+
+def _data(n=3000, seed=1):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, 8), np.float32)
+    X[:, :6] = rng.standard_normal((n, 6)).astype(np.float32)
+    T = rng.standard_normal(n).astype(np.float32)
+    return X, T
+
+
+def test_forward_and_gradient_vs_oracle():
+    _print("mlp forward / gradient vs FP64 oracle, n=3000")
+    rng = np.random.default_rng(3)
+    p = MO.init_params(5).astype(np.float32)
+    p[MO.OFF_B1:MO.OFF_W2] = rng.standard_normal(64) * 0.1
+    p[MO.OFF_B2:MO.OFF_W3] = rng.standard_normal(64) * 0.1
+    p[MO.OFF_B3] = 0.3
+    mlp = WarmStartMLP("cuda:0", params=p)
+    X, T = _data()
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    y = mlp.forward(Xd).cpu().numpy()
+    yref, _ = MO.forward(p.astype(np.float64), X)
+    assert np.abs(y - yref).max() <= 1e-5 * np.abs(yref).max() + 1e-6
+    loss, g = mlp.grad(Xd, Td)
+    g = g.cpu().numpy().astype(np.float64)
+    lref, gref = MO.loss_and_grad(p.astype(np.float64), X, T)
+    assert loss == pytest.approx(lref, rel=1e-4)
+    for lo, hi in ((MO.OFF_W1, MO.OFF_B1), (MO.OFF_B1, MO.OFF_W2), (MO.OFF_W2, MO.OFF_B2),
+                   (MO.OFF_B2, MO.OFF_W3), (MO.OFF_W3, MO.OFF_B3), (MO.OFF_B3, MO.N_PARAM)):
+        err = np.linalg.norm(g[lo:hi] - gref[lo:hi]) / max(np.linalg.norm(gref[lo:hi]), 1e-30)
+        _print(f"  block [{lo},{hi}) rel err {err:.2e}")
+        assert err <= 2e-3, (lo, hi, err)
+    _, g2 = mlp.grad(Xd, Td)
+    assert np.array_equal(g2.cpu().numpy(), g.astype(np.float32))
+
+
+def test_adam_reduces_loss():
+    _print("mlp adam fits a smooth target")
+    rng = np.random.default_rng(7)
+    X = np.zeros((20000, 8), np.float32)
+    X[:, :6] = rng.uniform(-1, 1, (20000, 6)).astype(np.float32)
+    T = (np.maximum(X[:, 0] + 0.5 * X[:, 1], 0) * 2.0 + 0.3 * X[:, 3] ** 2).astype(np.float32)
+    mlp = WarmStartMLP("cuda:0", seed=0)
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    l0, _ = mlp.grad(Xd, Td)
+    hist = mlp.train(Xd, Td, epochs=30, batch=4096, lr=3e-3)
+    _print(f"  loss {l0:.4f} -> {hist[-1]:.5f}")
+    assert hist[-1] < 0.05 * l0
+
+
+def _rollout(m, H, warm, steps, mlp=None, **kw):
+    s = SolverSPH(m, H, warm_start="zero" if warm == "mlp" else warm, **kw)
+    if mlp is not None:
+        s.set_mlp(mlp)
+    a, b = m.state(), m.state()
+    its = []
+    for _ in range(steps):
+        s.step(a, b, None, None, scenes.DT)
+        its.append(s.last_report["max_iterations"])
+        assert s.last_report["unconverged_worlds"] == 0
+        a, b = b, a
+    torch.cuda.synchronize()
+    return s, a, its
+
+
+def test_zero_network_equals_zero_warm_start():
+    cfg = scenes.small_dam_break(12, 10, 14)
+    _print("all-zero MLP warm start == zero warm start (bitwise), 6 steps")
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    mlp = WarmStartMLP("cuda:0", params=np.zeros(N_PARAM, np.float32))
+    _, a0, it0 = _rollout(m, cfg.support, "zero", 6)
+    _, a1, it1 = _rollout(m, cfg.support, "mlp", 6, mlp=mlp)
+    assert it0 == it1
+    assert np.array_equal(a0.particle_q.cpu().numpy(), a1.particle_q.cpu().numpy())
+
+
+def test_features_match_fields():
+    cfg = scenes.small_dam_break(12, 10, 14)
+    _print("mlp features vs solver fields")
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    s, st, _ = _rollout(m, cfg.support, "prev", 3)
+    X, T = features(s)
+    X, T = X.cpu().numpy(), T.cpu().numpy()
+    f = s.fields()
+    rho = f["density"].cpu().numpy()
+    p = f["pressure"].cpu().numpy()
+    cnt = f["neighbor_count"].cpu().numpy()
+    pb = O.problem_from_model(m, s)
+    ids, nf, nb = O.sorted_order(pb, st.particle_q.cpu().numpy())
+    # features are of the last step's input state, whose sorted order the oracle reproduces
+    # from the positions that entered the step; use the GPU's own order via a consistency check
+    assert X.shape == (nf, 8)
+    assert np.all(X[:, 6:] == 0)
+    assert np.all((X[:, 5] > 0) & (X[:, 5] <= 1.0))
+    assert np.all(X[:, 2] > 0)
+    fl = rho > 0
+    assert np.allclose(np.sort(X[:, 0]), np.sort((rho[fl] - 1000.0) / 1000.0 * 100.0), rtol=1e-5, atol=1e-4)
+    assert np.all(T >= 0)
+    assert np.isclose(np.count_nonzero(T > 0), np.count_nonzero(p[fl] > 0))
+
+
+def test_mlp_warm_start_converges_to_tolerance():
+    cfg = scenes.small_dam_break(14, 12, 16)
+    _print("random MLP warm start still meets the tolerance")
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    mlp = WarmStartMLP("cuda:0", seed=3)
+    s, a, its = _rollout(m, cfg.support, "mlp", 5, mlp=mlp)
+    assert s.last_report["max_density_error"] <= 1e-3
