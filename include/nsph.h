@@ -169,8 +169,13 @@ int nsph_solver_set_warm_start(nsph_solver* s, int mode);
 /* Forget the previous-step pressure (warm start input). */
 int nsph_solver_reset_pressure(nsph_solver* s);
 int nsph_solver_num_fluid(const nsph_solver* s, int* n_fluid, int* n_boundary);
+/* Workgroup size of the gather kernels (64, 128 or 256 threads; a 256-particle chunk is split
+ * over 256 / gather_block workgroups). Scheduling only: results are identical for every value.
+ * Set by the autotuner; captured graphs keep the value they were captured with. */
+int nsph_solver_set_launch(nsph_solver* s, int gather_block);
 /* Per-kernel-class GPU timing with HIP events recorded on the context stream around each
- * launch group (off by default; enabling resets the totals). nsph_solver_timing
+ * launch group of the classes whose bit is set in `enable` (bit k = class k; -1 = all; 0 = off,
+ * the default). Changing it resets the totals. nsph_solver_timing
  * synchronises and returns accumulated milliseconds and launch counts per class
  * (nsph_kernel_count() classes named by nsph_kernel_name). No reference counterpart:
  * the reference's timers are wall-clock ScopedTimer brackets (solver_xpbd.py:302,372). */
@@ -210,7 +215,8 @@ int nsph_mfma_selftest(nsph_ctx* ctx, const float* h_a, const float* h_b, float*
 /* Attach an MLP to a solver (NULL detaches); warm_start NSPH_WARM_MLP then predicts p0 from
  * the particle features before the Jacobi iterations, which still run to the tolerance. */
 int nsph_solver_set_mlp(nsph_solver* s, nsph_mlp* m);
-/* Features (n_fluid x 8) and target p / p_unit (n_fluid) of the last step, cell-sorted order. */
+/* Features (n_fluid x 8) and training target (p - 0.5 p_prev) / p_unit (n_fluid) of the last
+ * step, cell-sorted order; the MLP warm start is p0 = max(0.5 p_prev + y p_unit, 0). */
 int nsph_solver_mlp_features(nsph_solver* s, float* d_x, float* d_target);
 
 #ifdef __cplusplus

@@ -1,0 +1,110 @@
+"""Autotuner for the SPH step's launch configuration.
+
+Tunables (none of them changes a single bit of the results, which the GPU tests check):
+  gather_block   workgroup size of the gather kernels: a 256-particle chunk is split over
+                 1, 2 or 4 workgroups of 256 / 128 / 64 threads
+  mode           "eager" with a host convergence check every `check_every` iterations, or
+                 "graph": the step replayed from HIP graphs with `graph_slots` captured Jacobi
+                 iterations (one host read after the slots; eager iterations finish the solve
+                 if the slots were too few)
+The tuner times each candidate over a few steps of the caller's own solver and state
+(ping-pong, so the physics keeps advancing) and keeps the fastest. Results are persisted as
+JSON keyed by GPU architecture, fluid-count bucket and world count.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+DEFAULT_CACHE = Path(os.environ.get("NSPH_AUTOTUNE_CACHE",
+                                    Path(__file__).resolve().parent / "data" / "autotune.json"))
+
+
+def _arch() -> str:
+    try:
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    except Exception:
+        return "unknown"
+
+
+def workload_key(solver) -> str:
+    nf = max(solver.fluid_count, 1)
+    bucket = 1 << int(np.ceil(np.log2(nf)))
+    return f"{_arch()}|fluid<={bucket}|worlds={solver.W}|warm={solver.warm_start}"
+
+
+def candidates(mean_iterations: float) -> list[dict]:
+    k = max(2, int(np.ceil(mean_iterations)))
+    out = []
+    for gb in (256, 128, 64):
+        out.append({"gather_block": gb, "mode": "eager", "check_every": 1})
+        out.append({"gather_block": gb, "mode": "eager", "check_every": 2})
+        for slots in sorted({k, k + 1}):
+            out.append({"gather_block": gb, "mode": "graph", "graph_slots": slots})
+    return out
+
+
+def apply(solver, cfg: dict) -> None:
+    solver.set_launch(cfg["gather_block"])
+    if cfg["mode"] == "graph":
+        solver.use_graph = True
+        solver.graph_slots = cfg["graph_slots"]
+    else:
+        solver.use_graph = False
+        solver.check_every = cfg["check_every"]
+    solver._graphs.clear()
+
+
+def _time(solver, a, b, dt, steps) -> tuple[float, object, object]:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.step(a, b, None, None, dt)
+        a, b = b, a
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, a, b
+
+
+def tune(solver, state_a, state_b, dt: float, steps: int = 6, cache: Path | None = DEFAULT_CACHE,
+         force: bool = False, log=None) -> dict:
+    """Pick the fastest launch configuration for this solver and apply it. Returns the
+    chosen config with its measured ms per step (and the per-candidate table)."""
+    key = workload_key(solver)
+    table = {}
+    if cache is not None and Path(cache).exists():
+        try:
+            table = json.loads(Path(cache).read_text())
+        except ValueError:
+            table = {}
+    if not force and key in table:
+        apply(solver, table[key])
+        return table[key]
+    a, b = state_a, state_b
+    # iteration estimate from one eager step
+    solver.use_graph = False
+    solver.step(a, b, None, None, dt)
+    a, b = b, a
+    its = solver.last_report["max_iterations"] if solver.last_report else 2
+    results = []
+    for cfg in candidates(its):
+        apply(solver, cfg)
+        _, a, b = _time(solver, a, b, dt, 1)  # warm (captures graphs)
+        ms, a, b = _time(solver, a, b, dt, steps)
+        results.append({**cfg, "ms_per_step": ms * 1e3})
+        if log:
+            log(f"autotune {cfg} -> {ms * 1e3:.3f} ms")
+    best = min(results, key=lambda r: r["ms_per_step"])
+    chosen = {k: v for k, v in best.items()}
+    apply(solver, chosen)
+    chosen["candidates"] = results
+    if cache is not None:
+        table[key] = chosen
+        Path(cache).parent.mkdir(parents=True, exist_ok=True)
+        Path(cache).write_text(json.dumps(table, indent=1))
+    return chosen

@@ -90,6 +90,7 @@ __device__ inline float dot3(float ax, float ay, float az, float bx, float by, f
 
 struct SphDev {
     int Nf, cap;
+    int nsub;  // workgroups per chunk for the gather kernels (CHUNK / gather block size)
     // per-world tables
     const GridWorld* gw;
     const PhysWorld* phys;
@@ -128,11 +129,34 @@ struct SphDev {
     int* flags;  // [0] overflow, [1] max count, [2] non-finite
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. Renumber them
+// so that XCD x runs one contiguous range of logical workgroups: neighbouring chunks (which
+// gather each other's particles) then share an L2 instead of being spread over all eight.
+constexpr int NUM_XCD = 8;
+__device__ inline int xcd_swizzle(int i, int n) {
+    const int x = i % NUM_XCD, k = i / NUM_XCD;
+    const int q = n / NUM_XCD, r = n % NUM_XCD;
+    return x * q + min(x, r) + k;
+}
+
 // Map workgroup -> (world, rank inside world, sorted index). Returns false past the world's end.
 __device__ inline bool chunk_slot(const SphDev& D, int* w, int* s) {
-    const int c = blockIdx.x;
+    const int c = xcd_swizzle(blockIdx.x, gridDim.x);
     const int ww = __builtin_amdgcn_readfirstlane(D.chunk_world[c]);
     const int r = (c - D.w_cfirst[ww]) * CHUNK + (int)threadIdx.x;
+    *w = ww;
+    *s = D.w_fstart[ww] + r;
+    return r < D.w_nf[ww];
+}
+
+// Same with a chunk split over nsub workgroups of CHUNK/nsub threads (gather kernels without
+// a chunk reduction; the split changes scheduling only, never the arithmetic).
+__device__ inline bool chunk_slot_sub(const SphDev& D, int nsub, int* w, int* s) {
+    const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = b / nsub;
+    const int sub = b - c * nsub;
+    const int ww = __builtin_amdgcn_readfirstlane(D.chunk_world[c]);
+    const int r = (c - D.w_cfirst[ww]) * CHUNK + sub * (int)blockDim.x + (int)threadIdx.x;
     *w = ww;
     *s = D.w_fstart[ww] + r;
     return r < D.w_nf[ww];
@@ -328,19 +352,35 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 // vs[s].w so that the viscosity pass fetches velocity and density together.
 __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
-    if (!chunk_slot(D, &w, &s)) return;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const GridWorld g = D.gw[w];
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const int Nf = D.Nf, cap = D.cap;
-    int* __restrict__ out = reinterpret_cast<int*>(D.nbr4) + 4 * (size_t)s;
+    int4* __restrict__ out = D.nbr4 + s;
     int c = 0;
+    int4 quad = make_int4(0, 0, 0, 0);
+    // whole quads are stored (a partial 4-byte store per slot would write each 64-byte line
+    // back several times)
     rows_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
                [&](int j, float, float, float, float) {
                    if (j == s) return;
-                   if (c < cap) out[(size_t)(c >> 2) * 4 * Nf + (c & 3)] = j;
+                   if (c < cap) {
+                       const int u = c & 3;
+                       if (u == 0) quad.x = j;
+                       else if (u == 1) quad.y = j;
+                       else if (u == 2) quad.z = j;
+                       else {
+                           quad.w = j;
+                           out[(size_t)(c >> 2) * Nf] = quad;
+                       }
+                   }
                    ++c;
                });
+    {
+        const int cl = c < cap ? c : cap;
+        if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
+    }
     D.ncnt[s] = c < cap ? c : cap;
     if (c > cap) atomicOr(&D.flags[0], 1);
     atomicMax(&D.flags[1], c);
@@ -401,7 +441,7 @@ struct XA {
 //   v_adv  = v + dt (g + a_visc)
 __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
     int w, s;
-    if (!chunk_slot(D, &w, &s)) return;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
@@ -429,7 +469,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
 //   p_i = 0 | 0.5 p_prev | external;  pr_i = p_i / rho_i^2
 __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int warm) {
     int w, s;
-    if (!chunk_slot(D, &w, &s)) return;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const float4 vi = D.vadv[s];
@@ -478,7 +518,7 @@ __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int s, flo
 
 __global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
     int w, s;
-    const bool ok = chunk_slot(D, &w, &s);
+    const bool ok = chunk_slot_sub(D, D.nsub, &w, &s);
     if (D.conv[w]) return;
     if (!ok) return;
     const PhysWorld P = D.phys[w];
@@ -529,7 +569,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
         if (t < off) red[t] += red[t + off];
         __syncthreads();
     }
-    if (t == 0) D.partial[blockIdx.x] = red[0];
+    if (t == 0) D.partial[xcd_swizzle(blockIdx.x, gridDim.x)] = red[0];
 }
 
 // One wave per world: err_w = (sum of its chunk partials) / (n_w rho0); count the update;
@@ -568,7 +608,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
                                                      float* __restrict__ f_out,
                                                      float* __restrict__ pprev) {
     int w, s;
-    if (!chunk_slot(D, &w, &s)) return;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     float ax, ay, az;
@@ -619,7 +659,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
 //   f4 = 100 |v_adv| dt / H                       displacement per step, % of H
 //   f5 = distance to the nearest boundary particle / H (1 if none within H)
 // p_unit = 0.01 rho0 / (-a_ii) is the pressure that one Jacobi step assigns to a 1 % density
-// error; the network predicts p / p_unit and the warm start is p0 = max(y, 0) p_unit.
+// error.  The network predicts the correction to the IISPH previous-pressure warm start:
+// target (p - 0.5 p_prev) / p_unit, warm start p0 = max(0.5 p_prev + y p_unit, 0), so an
+// all-zero network reproduces the 0.5 p_prev warm start exactly.
 __device__ inline float mlp_features(const SphDev& D, const PhysWorld& P, int s, float dt,
                                      float* f) {
     const float rho = D.rho[s], src = D.src[s], a = D.aii[s];
@@ -637,7 +679,8 @@ __device__ inline float mlp_features(const SphDev& D, const PhysWorld& P, int s,
     return punit;
 }
 
-// Features and the training target (converged pressure / p_unit) of the last step, sorted order.
+// Features and the training target ((converged pressure - 0.5 p_prev) / p_unit) of the last
+// step, sorted order.
 __global__ __launch_bounds__(CHUNK) void k_sph_mlp_features(SphDev D, float dt, float* __restrict__ X,
                                                             float* __restrict__ T) {
     int w, s;
@@ -647,10 +690,11 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_features(SphDev D, float dt, 
     const float punit = mlp_features(D, P, s, dt, f);
 #pragma unroll
     for (int j = 0; j < MLP_XPAD; ++j) X[(size_t)s * MLP_XPAD + j] = f[j];
-    T[s] = punit > 0.0f ? D.p[s] / punit : 0.0f;
+    T[s] = punit > 0.0f ? (D.p[s] - 0.5f * D.pws[s]) / punit : 0.0f;
 }
 
-// Warm start: p0 = max(MLP(features), 0) p_unit, written to p and pr before the iterations.
+// Warm start: p0 = max(0.5 p_prev + MLP(features) p_unit, 0), written to p and pr before the
+// iterations.
 __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, float dt) {
     __shared__ __align__(16) float xs[CHUNK * MLP_XPAD];
     __shared__ __align__(16) __bf16 h1s[CHUNK / 64][16 * MLP_H];
@@ -682,7 +726,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
     }
     __syncthreads();
     if (ok) {
-        const float p0 = fmaxf(ys[t], 0.0f) * punit;
+        const float p0 = fmaxf(0.5f * D.pws[s] + ys[t] * punit, 0.0f);
         const float rho = D.rho[s];
         D.p[s] = p0;
         D.pr[s] = p0 / (rho * rho);
@@ -862,6 +906,7 @@ struct nsph_solver {
     double t_ms[KT_COUNT] = {0};
     long t_count[KT_COUNT] = {0};
     int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0, cap4 = 0;
+    int gblock = CHUNK;  // gather-kernel workgroup size (64, 128 or 256)
     int key_bits = 0;
     size_t total_cells = 0;
     double rho0 = 1000, omega = 0.5, tol = 1e-3;
@@ -896,6 +941,7 @@ static SphDev make_dev(nsph_solver* S) {
     SphDev D{};
     D.Nf = S->Nf;
     D.cap = S->cap;
+    D.nsub = CHUNK / S->gblock;
     D.gw = bp<GridWorld>(S->gw);
     D.phys = bp<PhysWorld>(S->phys);
     D.w_fstart = bp<int>(S->w_fstart);
@@ -966,7 +1012,7 @@ static void solver_free(nsph_solver* S) {
 // Timing brackets: record an event pair around a group of launches (timing mode only).
 static int tic(nsph_solver* S, int kind, EventPair** out) {
     *out = nullptr;
-    if (!S->timing) return NSPH_OK;
+    if (!((S->timing >> kind) & 1)) return NSPH_OK;
     if (S->ev_used == S->ev_pool.size()) {
         EventPair e{};
         NSPH_HIP(hipEventCreate(&e.a));
@@ -1280,9 +1326,10 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
                        S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
                        Nf, D.xs, D.vs, D.pws, D.ftable);
     NSPH_TRY(toc(S, ep_build));
-    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, dim3(C), dim3(CHUNK), 0, st, D, dt2));
-    TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, dim3(C), dim3(CHUNK), 0, st, D, dt));
-    TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, dim3(C), dim3(CHUNK), 0, st, D, dt, S->warm));
+    const dim3 gg(C * D.nsub), gb(S->gblock);
+    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));
+    TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt));
+    TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_MLP, hipLaunchKernelGGL(k_sph_mlp_warm, dim3(C), dim3(CHUNK), 0, st, D,
                                          nsph_mlp_device_view(S->mlp), dt));
@@ -1302,7 +1349,7 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
     int launched = it0;
     if (S->Nf > 0) {
         for (int it = it0; it < it1; ++it) {
-            TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C), dim3(CHUNK), 0, st, D));
+            TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
             TIMED(KT_PUPDATE, hipLaunchKernelGGL(k_sph_pupdate, dim3(C), dim3(CHUNK), 0, st, D, dt2,
                                                  (float)S->omega));
             TIMED(KT_REDUCE, hipLaunchKernelGGL(k_sph_reduce, dim3(red_blocks), dim3(256), 0, st, D,
@@ -1327,7 +1374,7 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, c
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
     if (S->Nf > 0)
-        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C), dim3(CHUNK), 0, st, D, S->perm,
+        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, S->perm,
                                            bp<int>(S->fid), S->d_inv_mass, dt, S->v_max, q_out,
                                            qd_out, f_out, bp<float>(S->pprev)));
     if (S->No > 0) {
@@ -1554,10 +1601,18 @@ int nsph_solver_mlp_features(nsph_solver* S, float* d_x, float* d_target) {
     return NSPH_OK;
 }
 
+int nsph_solver_set_launch(nsph_solver* S, int gather_block) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_launch: NULL solver");
+    NSPH_CHECK(gather_block == 64 || gather_block == 128 || gather_block == 256, NSPH_ERR_INVALID,
+               "nsph_solver_set_launch: gather_block must be 64, 128 or 256 (got %d)", gather_block);
+    S->gblock = gather_block;
+    return NSPH_OK;
+}
+
 int nsph_solver_set_timing(nsph_solver* S, int enable) {
     NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_timing: NULL solver");
     NSPH_TRY(flush_timing(S));
-    S->timing = enable ? 1 : 0;
+    S->timing = enable;
     for (int k = 0; k < KT_COUNT; ++k) {
         S->t_ms[k] = 0.0;
         S->t_count[k] = 0;

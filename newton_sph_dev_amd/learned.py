@@ -1,11 +1,12 @@
 """Learned warm start for the IISPH pressure solve.
 
 A per-particle MLP 6 -> 64 -> 64 -> 1 (BF16 weights, FP32 accumulation, hand-written gfx950
-MFMA kernels in libnsph: forward, gradient, Adam) predicts p / p_unit from
+MFMA kernels in libnsph: forward, gradient, Adam) predicts the correction
+(p - 0.5 p_prev) / p_unit to the IISPH previous-pressure warm start from
   density error, velocity divergence, neighbour count, previous pressure, |v| dt / H and
   distance to the nearest boundary particle
-(exact definitions in csrc/sph.hip, mlp_features).  The solver uses it as the initial
-pressure and always finishes with the plain Jacobi iterations to the same tolerance, so
+(exact definitions in csrc/sph.hip, mlp_features).  The solver starts the Jacobi iterations
+from p0 = max(0.5 p_prev + y p_unit, 0) and always finishes with the plain Jacobi iterations to the same tolerance, so
 the plain solver stays the fallback and the judge of convergence.
 
 Training data are solver snapshots: after each step of seeded rollouts of the dam-break
@@ -95,18 +96,25 @@ class WarmStartMLP:
         N.call("nsph_mlp_adam", self._handle, C.c_float(lr), C.c_float(beta1), C.c_float(beta2), C.c_float(eps))
 
     def train(self, X: torch.Tensor, T: torch.Tensor, epochs: int = 10, batch: int = 65536,
-              lr: float = 2e-3, seed: int = 0, log=None) -> list[float]:
-        """Adam over shuffled mini-batches; returns the mean loss per epoch."""
+              lr: float = 2e-3, lr_min: float = 5e-5, seed: int = 0, log=None) -> list[float]:
+        """Adam over shuffled mini-batches with a cosine learning-rate decay from lr to
+        lr_min; returns the mean loss per epoch."""
+        import math
+
         gen = torch.Generator(device="cpu").manual_seed(seed)
         n = X.shape[0]
         hist = []
+        total = epochs * ((n + batch - 1) // batch)
+        it = 0
         for ep in range(epochs):
             perm = torch.randperm(n, generator=gen).to(self.device)
             tot, cnt = 0.0, 0
             for b0 in range(0, n, batch):
                 idx = perm[b0:b0 + batch]
                 loss, _ = self.grad(X[idx], T[idx])
-                self.adam(lr)
+                rate = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * it / max(total - 1, 1)))
+                self.adam(rate)
+                it += 1
                 tot += loss * idx.numel()
                 cnt += idx.numel()
             hist.append(tot / max(cnt, 1))
@@ -132,7 +140,7 @@ class WarmStartMLP:
 
 
 def features(solver: SolverSPH) -> tuple[torch.Tensor, torch.Tensor]:
-    """(X, T) of the solver's last step: features (Nf, 8) and target p / p_unit (Nf,)."""
+    """(X, T) of the solver's last step: features (Nf, 8) and target (p - 0.5 p_prev) / p_unit."""
     dev = solver.model.device
     nf = solver.fluid_count
     X = torch.empty((nf, X_STRIDE), dtype=torch.float32, device=dev)
@@ -148,7 +156,7 @@ class SnapshotConfig:
     spacing: float = 0.0125
     extent: tuple[float, float] = (0.1, 0.3)
     seed: int = 2024
-    dt: float = scenes.DT
+    dts: tuple = (scenes.DT,)
     tolerance: float = 1e-4
 
 
@@ -181,18 +189,20 @@ def collect_snapshots(cfg: SnapshotConfig, device="cuda:0", keep_every: int = 1,
     Returns X (n, 8), T (n,) on the device and the number of world snapshots."""
     b, H = build_training_worlds(cfg)
     m = b.finalize(device)
-    solver = SolverSPH(m, H, warm_start="prev", tolerance=cfg.tolerance, max_iters=200)
-    a, c = m.state(), m.state()
     Xs, Ts = [], []
     snaps = 0
-    for k in range(cfg.steps):
-        solver.step(a, c, None, None, cfg.dt)
-        a, c = c, a
-        if k % keep_every == 0:
-            X, T = features(solver)
-            Xs.append(X)
-            Ts.append(T)
-            snaps += cfg.worlds
+    steps = max(1, cfg.steps // len(cfg.dts))
+    for dt in cfg.dts:
+        solver = SolverSPH(m, H, warm_start="prev", tolerance=cfg.tolerance, max_iters=300)
+        a, c = m.state(), m.state()
+        for k in range(steps):
+            solver.step(a, c, None, None, dt)
+            a, c = c, a
+            if k % keep_every == 0:
+                X, T = features(solver)
+                Xs.append(X)
+                Ts.append(T)
+                snaps += cfg.worlds
     X = torch.cat(Xs)
     T = torch.cat(Ts)
     if max_samples is not None and X.shape[0] > max_samples:
@@ -253,7 +263,8 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--load", default="")
     ap.add_argument("--eval-steps", type=int, default=30)
-    ap.add_argument("--eval-dt", type=float, nargs="*", default=[scenes.DT])
+    ap.add_argument("--eval-dt", type=float, nargs="*", default=[scenes.DT, 5e-4, 1e-3])
+    ap.add_argument("--train-dt", type=float, nargs="*", default=[scenes.DT, 5e-4, 1e-3])
     ap.add_argument("--min-iters", type=int, nargs="*", default=[2, 1])
     args = ap.parse_args(argv)
     log = lambda msg: print(f"[learned] {msg}", file=sys.stderr, flush=True)  # noqa: E731
@@ -261,7 +272,7 @@ def main(argv=None):
     if args.load:
         mlp = WarmStartMLP.load(args.load)
     else:
-        cfg = SnapshotConfig(worlds=args.worlds, steps=args.steps)
+        cfg = SnapshotConfig(worlds=args.worlds, steps=args.steps, dts=tuple(args.train_dt))
         t0 = time.perf_counter()
         X, T, snaps = collect_snapshots(cfg, max_samples=args.samples)
         report["snapshots"] = snaps
@@ -282,8 +293,11 @@ def main(argv=None):
         for mi in args.min_iters:
             key = f"dam_break_1m dt={dt:g} min_iters={mi}"
             log(f"evaluate {key}")
-            report[key] = evaluate(lambda: scenes.build_dam_break(cfg1m), cfg1m.support, mlp,
-                                   steps=args.eval_steps, dt=dt, min_iters=mi)
+            try:
+                report[key] = evaluate(lambda: scenes.build_dam_break(cfg1m), cfg1m.support, mlp,
+                                       steps=args.eval_steps, dt=dt, min_iters=mi)
+            except N.NsphError as e:
+                report[key] = {"error": str(e)}
     print(json.dumps(report))
 
 

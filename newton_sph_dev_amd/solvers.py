@@ -159,7 +159,8 @@ class SolverSPH(SolverBase):
                  boundary_volume: float | Sequence[float] | None = None,
                  domain: tuple | None = None, omega: float = 0.5, tolerance: float = 1.0e-3,
                  min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 64,
-                 warm_start: str = "zero", check_every: int = 1, report: bool = True):
+                 warm_start: str = "zero", check_every: int = 1, report: bool = True,
+                 gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None):
         super().__init__(model)
         if model.device.type != "cuda":
             raise RuntimeError("SolverSPH runs on a ROCm GPU only (model.device is %s)" % model.device)
@@ -177,6 +178,10 @@ class SolverSPH(SolverBase):
         self.check_every = int(check_every)
         self.report = bool(report)
         self.last_report: dict | None = None
+        self.gather_block = int(gather_block)
+        self.use_graph = bool(use_graph)
+        self.graph_slots = graph_slots
+        self._graphs: dict = {}
         self._handle = None
         self._create()
 
@@ -223,6 +228,7 @@ class SolverSPH(SolverBase):
         h = C.c_void_p()
         N.call("nsph_solver_create", self._ctx, C.byref(md), C.byref(pr), C.byref(h))
         self._handle = h
+        N.call("nsph_solver_set_launch", h, self.gather_block)
         nf, nb = C.c_int(), C.c_int()
         N.call("nsph_solver_num_fluid", h, C.byref(nf), C.byref(nb))
         self.fluid_count = nf.value
@@ -259,10 +265,23 @@ class SolverSPH(SolverBase):
         for t in (state_in.particle_q, state_in.particle_qd, state_out.particle_q, state_out.particle_qd):
             if t is None or t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous():
                 raise ValueError("state arrays must be contiguous float32 tensors on the model device")
+        if self.use_graph:
+            key = (state_in.particle_q.data_ptr(), state_in.particle_qd.data_ptr(),
+                   state_out.particle_q.data_ptr(), state_out.particle_qd.data_ptr(),
+                   None if state_out.particle_f is None else state_out.particle_f.data_ptr(), float(dt))
+            h = self._graphs.get(key)
+            if h is None:
+                h = self.capture(state_in, state_out, dt, self.graph_slots)
+                self._graphs[key] = h
+            self.replay(h)
+            return
         rep = N.StepReport() if self.report else None
         N.call("nsph_solver_step", self._handle, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
                N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(state_out.particle_f),
                float(dt), self.check_every, C.byref(rep) if rep is not None else None)
+        self._check_report(rep)
+
+    def _check_report(self, rep) -> None:
         if rep is not None:
             self.last_report = rep.as_dict()
             if rep.status == N.NSPH_ERR_CAPACITY:
@@ -293,8 +312,13 @@ class SolverSPH(SolverBase):
         N.set_stream(self._ctx, torch.cuda.current_stream(self.model.device).cuda_stream)
         rep = N.StepReport() if self.report else None
         N.call("nsph_solver_graph_step", self._handle, int(handle), C.byref(rep) if rep is not None else None)
-        if rep is not None:
-            self.last_report = rep.as_dict()
+        self._check_report(rep)
+
+    def set_launch(self, gather_block: int) -> None:
+        """Gather-kernel workgroup size (64/128/256); results do not depend on it."""
+        N.call("nsph_solver_set_launch", self._handle, int(gather_block))
+        self.gather_block = int(gather_block)
+        self._graphs.clear()
 
     def collect_report(self) -> dict:
         rep = N.StepReport()

@@ -86,3 +86,36 @@ def test_graph_replay_equals_eager(slots):
     assert np.array_equal(a.particle_q.cpu().numpy(), ste.particle_q.cpu().numpy())
     assert np.array_equal(a.particle_qd.cpu().numpy(), ste.particle_qd.cpu().numpy())
     assert sg.last_report["unconverged_worlds"] == 0
+
+
+def test_launch_configs_do_not_change_results():
+    cfg = scenes.small_batch(10, seed=5)
+    _print("gather block 64/128/256 and graph mode give identical steps")
+    b, H = scenes.build_batched(cfg)
+    m = b.finalize("cuda:0")
+    outs = []
+    for gb, graph in ((256, False), (128, False), (64, False), (128, True)):
+        s = SolverSPH(m, H, warm_start="prev", gather_block=gb, use_graph=graph, graph_slots=2)
+        a, c = m.state(), m.state()
+        for _ in range(4):
+            s.step(a, c, None, None, scenes.DT)
+            a, c = c, a
+        torch.cuda.synchronize()
+        outs.append((a.particle_q.cpu().numpy(), a.particle_qd.cpu().numpy()))
+    for q, qd in outs[1:]:
+        assert np.array_equal(q, outs[0][0]) and np.array_equal(qd, outs[0][1])
+
+
+def test_autotune_picks_a_config(tmp_path):
+    from newton_sph_dev_amd import autotune
+
+    cfg = scenes.small_dam_break(16, 12, 16)
+    _print("autotune on a small dam break")
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    s = SolverSPH(m, cfg.support, warm_start="prev")
+    a, c = m.state(), m.state()
+    best = autotune.tune(s, a, c, scenes.DT, steps=2, cache=tmp_path / "tune.json")
+    assert best["gather_block"] in (64, 128, 256) and best["mode"] in ("eager", "graph")
+    assert (tmp_path / "tune.json").exists()
+    again = autotune.tune(s, a, c, scenes.DT, cache=tmp_path / "tune.json")
+    assert again["gather_block"] == best["gather_block"]

@@ -5,7 +5,8 @@ Bars: MFMA operand layout exact on a BF16-representable product; forward within 
 (relative to max |y|) of the BF16-emulating FP64 oracle; gradient within 2e-3 norm-wise
 (BF16 roundings of intermediates can flip by one unit where FP32 and FP64 sums straddle a
 rounding boundary); bitwise repeatable gradients; an all-zero network reproduces the zero
-warm start bit for bit; the MLP warm start meets the same tolerance as the plain solver.
+an all-zero network reproduces the previous-pressure warm start bit for bit (the network
+predicts a correction to it); the MLP warm start meets the same tolerance as the plain solver.
 """
 
 from __future__ import annotations
@@ -104,12 +105,12 @@ def _rollout(m, H, warm, steps, mlp=None, **kw):
     return s, a, its
 
 
-def test_zero_network_equals_zero_warm_start():
+def test_zero_network_equals_prev_warm_start():
     cfg = scenes.small_dam_break(12, 10, 14)
-    _print("all-zero MLP warm start == zero warm start (bitwise), 6 steps")
+    _print("all-zero MLP warm start == 0.5 p_prev warm start (bitwise), 6 steps")
     m = scenes.build_dam_break(cfg).finalize("cuda:0")
     mlp = WarmStartMLP("cuda:0", params=np.zeros(N_PARAM, np.float32))
-    _, a0, it0 = _rollout(m, cfg.support, "zero", 6)
+    _, a0, it0 = _rollout(m, cfg.support, "prev", 6)
     _, a1, it1 = _rollout(m, cfg.support, "mlp", 6, mlp=mlp)
     assert it0 == it1
     assert np.array_equal(a0.particle_q.cpu().numpy(), a1.particle_q.cpu().numpy())
@@ -136,8 +137,7 @@ def test_features_match_fields():
     assert np.all(X[:, 2] > 0)
     fl = rho > 0
     assert np.allclose(np.sort(X[:, 0]), np.sort((rho[fl] - 1000.0) / 1000.0 * 100.0), rtol=1e-5, atol=1e-4)
-    assert np.all(T >= 0)
-    assert np.isclose(np.count_nonzero(T > 0), np.count_nonzero(p[fl] > 0))
+    assert np.all(np.isfinite(T))
 
 
 def test_mlp_warm_start_converges_to_tolerance():

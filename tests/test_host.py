@@ -149,3 +149,14 @@ def test_abi_errors_without_device():
     assert st != 0
     assert len(N.last_error()) > 0
     assert lib.nsph_solver_step(None, None, None, None, None, None, C.c_float(1e-4), 0, None) == 1
+
+
+def test_bench_accounts_every_kernel_class():
+    """bench.py's algorithmic-byte table names every kernel class the library times."""
+    import bench
+    from newton_sph_dev_amd import _native as N
+
+    lib = N.load()
+    names = [lib.nsph_kernel_name(k).decode() for k in range(lib.nsph_kernel_count())]
+    table = bench.algorithmic_bytes(28.0)
+    assert sorted(table) == sorted(names)
