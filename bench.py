@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay captured HIP graphs")
+    ap.add_argument("--graph-slots", type=int, default=0,
+                    help="Jacobi iterations captured in the graph (0: warm-up maximum + 1)")
     ap.add_argument("--autotune", action="store_true", help="tune the launch configuration first")
     ap.add_argument("--mlp", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                   "newton_sph_dev_amd", "data", "warmstart_mlp.npy"),
@@ -122,7 +124,7 @@ def main():
         model = scenes.build_dam_break(cfg).finalize("cuda:0")
         support = cfg.support
     solver = SolverSPH(model, support, warm_start="zero" if args.warm_start == "mlp" else args.warm_start,
-                       check_every=args.check_every, use_graph=args.graph)
+                       check_every=args.check_every)
     if args.warm_start == "mlp":
         from newton_sph_dev_amd.learned import WarmStartMLP
 
@@ -133,10 +135,15 @@ def main():
     dt = scenes.DT
     print(f"[bench] fluid={solver.fluid_count} boundary={solver.boundary_count}; warmup {args.warmup}",
           file=sys.stderr, flush=True)
+    warm_iters = []
     for _ in range(args.warmup):
         solver.step(s0, s1, None, None, dt)
+        warm_iters.append(solver.last_report["max_iterations"])
         s0, s1 = s1, s0
     torch.cuda.synchronize()
+    if args.graph:
+        solver.use_graph = True
+        solver.graph_slots = args.graph_slots or (max(warm_iters, default=2) + 1)
     tuned = None
     if args.autotune:
         from newton_sph_dev_amd import autotune
@@ -158,12 +165,16 @@ def main():
     names = [lib.nsph_kernel_name(k).decode() for k in range(nk)]
     ms = (C.c_double * nk)()
     cnt = (C.c_int * nk)()
-    # short pass with every class timed, to find the dominant kernel class
+    # short eager pass with every class timed, to find the dominant kernel class (graph
+    # replays carry no per-kernel events)
+    graph_mode = solver.use_graph
+    solver.use_graph = False
     N.call("nsph_solver_set_timing", solver._handle, -1)
     for _ in range(3):
         solver.step(s0, s1, None, None, dt)
         s0, s1 = s1, s0
     N.call("nsph_solver_timing", solver._handle, ms, cnt)
+    solver.use_graph = graph_mode
     per_all = {names[k]: (ms[k], cnt[k]) for k in range(nk)}
     counts = solver.fields()["neighbor_count"]
     nf = solver.fluid_count
@@ -187,6 +198,12 @@ def main():
     N.call("nsph_solver_timing", solver._handle, ms, cnt)
     N.call("nsph_solver_set_timing", solver._handle, 0)
     per = {names[k]: (ms[k], cnt[k]) for k in range(nk)}
+    roofline_src = "HIP events around every launch of the class in the timed steps"
+    if graph_mode or per[dom][1] == 0 or per[dom][0] <= 0.0:
+        # graph replays carry no per-kernel events (only the eager finish of a solve whose
+        # captured slots were too few does): use the eager all-class pass
+        per = per_all
+        roofline_src = "HIP events of a 3-step eager pass (the timed steps replay HIP graphs)"
     avg_s = per[dom][0] / max(per[dom][1], 1) / 1e3
     achieved = ab[dom] * nf / avg_s / 1e9
     value = nf * args.steps / el
@@ -232,10 +249,12 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(cfg.name, dom),
             "algorithmic_bytes_per_launch": ab[dom] * nf,
+            "source": roofline_src,
         },
         "step_hbm_frac": step_bytes / (el / args.steps) / 1e9 / HBM_PEAK_GBS,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "launch": tuned or {"gather_block": solver.gather_block, "graph": solver.use_graph,
+                            "graph_slots": solver.graph_slots if solver.use_graph else None,
                             "check_every": solver.check_every},
         "kernels": kern,
     }

@@ -87,6 +87,32 @@ int nsph_grid_query(const nsph_grid* grid, const float* d_q, int nq, float radiu
                     int max_neighbors, int* d_counts, int* d_idx);
 int nsph_grid_destroy(nsph_grid* grid);
 
+/* ---- the reference's particle kernels on the HIP grid ------------------------------ */
+
+/* integrate_particles (newton/_src/solvers/solver.py:10-47, launched at solver.py:266-299):
+ * d_f may be NULL (zero force); d_gravity is a device array of n_gravity vec3 (model.gravity). */
+int nsph_particles_integrate(nsph_ctx* ctx, int n, const float* d_x, const float* d_v, const float* d_f,
+                             const float* d_inv_mass, const int* d_flags, const int* d_world,
+                             const float* d_gravity, int n_gravity, float dt, float v_max,
+                             float* d_x_out, float* d_v_out);
+/* solve_particle_particle_contacts (newton/_src/solvers/xpbd/kernels.py:225-289, launched at
+ * solver_xpbd.py:420-441) over a grid built from the same n particles; adds each active
+ * particle's correction into d_deltas[i] (the caller zeroes it, solver_xpbd.py:379-383). */
+int nsph_xpbd_particle_contacts(nsph_grid* grid, int n, const float* d_x, const float* d_v,
+                                const float* d_inv_mass, const float* d_radius, const int* d_flags,
+                                float mu, float cohesion, float max_radius, float dt, float relaxation,
+                                float* d_deltas);
+/* apply_particle_deltas (newton/_src/solvers/xpbd/kernels.py:776-806). */
+int nsph_apply_particle_deltas(nsph_ctx* ctx, int n, const float* d_x_orig, const float* d_x_pred,
+                               const int* d_flags, const float* d_deltas, float dt, float v_max,
+                               float* d_x_out, float* d_v_out);
+/* eval_particle_contact (newton/_src/solvers/semi_implicit/kernels_contact.py:35-84, launched at
+ * kernels_contact.py:559-579) over a grid built from the same n particles: overwrites d_f[i]
+ * of active particles, as the reference does. */
+int nsph_eval_particle_contact(nsph_grid* grid, int n, const float* d_x, const float* d_v, const float* d_radius,
+                               const int* d_flags, float ke, float kd, float kf, float mu, float cohesion,
+                               float max_radius, float* d_f);
+
 /* ---- SPH solver (behind SolverBase.step, solver.py:301-316) ------------------------ */
 
 enum nsph_warm_start { NSPH_WARM_ZERO = 0, NSPH_WARM_PREV = 1, NSPH_WARM_MLP = 2 };

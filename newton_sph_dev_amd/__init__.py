@@ -6,6 +6,7 @@ declared in include/nsph.h) on PyTorch-ROCm tensors.
 
 from .hashgrid import HashGrid
 from .sim import Axis, Contacts, Control, Model, ModelBuilder, ParticleFlags, State
+from .particle_solvers import SolverSemiImplicit, SolverXPBD
 from .solvers import SolverBase, SolverNotifyFlags, SolverSPH
 
 __all__ = [
@@ -19,5 +20,7 @@ __all__ = [
     "SolverBase",
     "SolverNotifyFlags",
     "SolverSPH",
+    "SolverSemiImplicit",
+    "SolverXPBD",
     "State",
 ]

@@ -126,6 +126,10 @@ SIGNATURES: dict[str, list] = {
     "nsph_kernel_count": [],
     "nsph_solver_iterations_launched": [_vp, _pi],
     "nsph_solver_destroy": [_vp],
+    "nsph_particles_integrate": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp, _vp],
+    "nsph_xpbd_particle_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _vp],
+    "nsph_apply_particle_deltas": [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp],
+    "nsph_eval_particle_contact": [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f, _vp],
     "nsph_mlp_num_params": [],
     "nsph_mlp_create": [_vp, C.POINTER(_vp)],
     "nsph_mlp_destroy": [_vp],

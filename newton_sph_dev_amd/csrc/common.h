@@ -186,7 +186,19 @@ __global__ void k_fill_i32(int* p, size_t n, int v);
 
 int bits_for(uint32_t max_key);
 
+// Read-only view of a built HashGrid for consumer kernels (cell table, sorted point ids).
+struct GridView {
+    GridWorld g;
+    const int2* table;
+    const uint32_t* perm;
+    int n;
+};
+
 }  // namespace nsph
+
+struct nsph_grid;
+int nsph_grid_view(const nsph_grid* g, nsph::GridView* out);
+int nsph_grid_stream(const nsph_grid* g, hipStream_t* out);
 
 struct nsph_ctx {
     int device = 0;

@@ -150,6 +150,22 @@ struct nsph_grid {
     RadixSort rs;
 };
 
+int nsph_grid_view(const nsph_grid* g, GridView* out) {
+    NSPH_CHECK(g != nullptr && out != nullptr, NSPH_ERR_INVALID, "grid view: NULL grid");
+    NSPH_CHECK(g->built, NSPH_ERR_NOT_BUILT, "grid view: the grid has not been built");
+    out->g = g->g;
+    out->table = bp<int2>(g->table);
+    out->perm = g->perm;
+    out->n = g->n;
+    return NSPH_OK;
+}
+
+int nsph_grid_stream(const nsph_grid* g, hipStream_t* out) {
+    NSPH_CHECK(g != nullptr && out != nullptr, NSPH_ERR_INVALID, "grid stream: NULL grid");
+    *out = g->ctx->stream;
+    return NSPH_OK;
+}
+
 extern "C" {
 
 const char* nsph_last_error(void) { return g_err; }

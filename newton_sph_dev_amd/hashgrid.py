@@ -23,6 +23,7 @@ class HashGrid:
         self.dim_x, self.dim_y, self.dim_z = int(dim_x), int(dim_y), int(dim_z)
         self.device = torch.device(device) if device is not None else torch.device("cuda:0")
         self.reserved = 0
+        self.built = False
         self.radius = 0.0
         self._points = None
         self._handle = None
@@ -59,6 +60,7 @@ class HashGrid:
         N.call("nsph_grid_build", self._handle, N.ptr(pts), int(pts.shape[0]), float(radius))
         self.reserved = max(self.reserved, int(pts.shape[0]))
         self.radius = float(radius)
+        self.built = True
         self._points = pts
 
     @property

@@ -27,9 +27,28 @@ class ParticleFlags(enum.IntEnum):
 
 
 class Axis(enum.IntEnum):
+    """Axes of 3D space (newton/_src/core/types.py:81-130)."""
+
     X = 0
     Y = 1
     Z = 2
+
+    @classmethod
+    def from_string(cls, axis_str: str) -> "Axis":
+        a = axis_str.lower()
+        if a in ("x", "y", "z"):
+            return cls("xyz".index(a))
+        raise ValueError(f"Invalid axis string: {axis_str}")
+
+    @classmethod
+    def from_any(cls, value) -> "Axis":
+        if isinstance(value, cls):
+            return value
+        if isinstance(value, str):
+            return cls.from_string(value)
+        if isinstance(value, (int, np.integer)):
+            return cls(int(value))
+        raise TypeError(f"Cannot convert {type(value).__name__} to Axis")
 
 
 def quat_to_matrix(q: Sequence[float]) -> np.ndarray:
@@ -52,8 +71,8 @@ def _identity_quat(q: Sequence[float]) -> bool:
 class ModelBuilder:
     """Builds a particle Model (particle methods of newton/_src/sim/builder.py)."""
 
-    def __init__(self, up_axis: Axis | int = Axis.Z, gravity: float = -9.81):
-        self.up_axis = Axis(int(up_axis))
+    def __init__(self, up_axis: Axis | int | str = Axis.Z, gravity: float = -9.81):
+        self.up_axis = Axis.from_any(up_axis)
         self.gravity = float(gravity)
         self.world_count = 0
         self.world_gravity: list[tuple[float, float, float]] = []
@@ -280,6 +299,30 @@ class Model:
 
     def contacts(self) -> Contacts:
         return Contacts()
+
+    def set_gravity(self, gravity, world: int | None = None) -> None:
+        """Runtime gravity change (model.py:923-958): one vector for every world, one per
+        world ((world_count, 3)), or one vector for `world` only.  Solvers that keep a copy
+        need notify_model_changed(SolverNotifyFlags.MODEL_PROPERTIES) afterwards."""
+        g = np.asarray(gravity, dtype=np.float32)
+        if world is not None:
+            if g.shape != (3,):
+                raise ValueError("Expected single gravity vector (3,) when world is specified")
+            if world < 0 or world >= self.world_count:
+                raise IndexError(f"world {world} out of range [0, {self.world_count})")
+            self.gravity[world] = torch.from_numpy(g).to(self.gravity.device)
+        elif g.ndim == 1:
+            self.gravity[:] = torch.from_numpy(g.reshape(1, 3)).to(self.gravity.device)
+        else:
+            if len(g) != self.world_count:
+                raise ValueError(f"Expected {self.world_count} gravity vectors, got {len(g)}")
+            self.gravity.copy_(torch.from_numpy(g.reshape(-1, 3)).to(self.gravity.device))
+
+    def collide(self, state: State, contacts: Contacts | None = None, **kwargs) -> Contacts:
+        """Contact generation (model.py:996-1030).  A particle-only model has no shapes, so
+        there are no particle-shape or rigid contacts to generate; particle-particle contacts
+        are found by the solvers through particle_grid."""
+        return contacts if contacts is not None else self.contacts()
 
 
 class State:

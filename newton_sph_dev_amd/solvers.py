@@ -52,6 +52,21 @@ class SolverBase:
     def update_contacts(self, contacts: Contacts, state: State | None = None) -> None:
         raise NotImplementedError()
 
+    def integrate_particles(self, model: Model, state_in: State, state_out: State, dt: float) -> None:
+        """Semi-implicit Euler of the particles (solver.py:266-299, kernel solver.py:10-47):
+        v1 = v0 + (f w + g step(-w)) dt clamped to particle_max_velocity, x1 = x0 + v1 dt;
+        inactive particles copy x and leave v_out untouched."""
+        n = model.particle_count
+        if not n:
+            return
+        ctx = N.context(model.device.index or 0)
+        N.set_stream(ctx, torch.cuda.current_stream(model.device).cuda_stream)
+        g = model.gravity.reshape(-1, 3).contiguous()
+        N.call("nsph_particles_integrate", ctx, n, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
+               N.ptr(state_in.particle_f), N.ptr(model.particle_inv_mass), N.ptr(model.particle_flags),
+               N.ptr(model.particle_world), N.ptr(g), int(g.shape[0]), float(dt),
+               float(model.particle_max_velocity), N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd))
+
     @classmethod
     def register_custom_attributes(cls, builder: ModelBuilder) -> None:
         pass

@@ -1,0 +1,218 @@
+"""The reference's HashGrid consumers (XPBD particle contacts, semi-implicit penalty contacts,
+particle integration) on the HIP grid, against the FP64 oracle and the reference's own tests.
+
+Bars: each kernel and each solver step within FP32 rounding of the FP64 oracle (absolute
+tolerances below, positions of order 1); the two particle-particle friction cases of
+newton/tests/test_solver_xpbd.py:20-225 with the reference's own assertions.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from newton_sph_dev_amd import ModelBuilder, SolverSemiImplicit, SolverXPBD
+from newton_sph_dev_amd import _native as N
+from oracle import contacts_oracle as K
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _pile(n_side=12, radius=0.05, seed=0, inactive_every=17, static_every=23):
+    """Lattice of spacing 1.8 r jittered by up to 0.05 r per axis: axis neighbours overlap by
+    0.1 r .. 0.3 r, diagonal ones are 0.4 r apart.  Returns a finalized model after checking
+    that no pair sits within 1e-4 of the contact threshold, so FP32 and FP64 agree on which
+    pairs touch (the penalty force jumps there through its damping term; the XPBD
+    correction is continuous at the threshold when cohesion is 0)."""
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = g * (1.8 * radius) + rng.uniform(-0.05 * radius, 0.05 * radius, g.shape)
+    v = rng.normal(0.0, 0.5, g.shape)
+    n = len(x)
+    mass = np.full(n, 0.01)
+    mass[::static_every] = 0.0
+    v[::static_every] = 0.0
+    b = ModelBuilder(up_axis="Y")
+    b.add_particles(pos=x.tolist(), vel=v.tolist(), mass=mass.tolist(), radius=[radius] * n)
+    m = b.finalize(DEV)
+    flags = m.particle_flags.clone()
+    flags[::inactive_every] = 0
+    m.particle_flags = flags
+    d = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=-1)
+    err = d - 2 * radius
+    np.fill_diagonal(err, 1.0)
+    assert np.min(np.abs(err)) > 1e-4, "re-seed: a pair sits on the contact threshold"
+    return m
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _host(m):
+    return dict(inv_mass=_np(m.particle_inv_mass), radius=_np(m.particle_radius),
+                flags=m.particle_flags.cpu().numpy(), world=m.particle_world.cpu().numpy(),
+                gravity=_np(m.gravity))
+
+
+def test_xpbd_contact_kernel_matches_oracle():
+    m = _pile()
+    n = m.particle_count
+    _print(f"xpbd contact kernel, {n} particles")
+    s = m.state()
+    grid = m.particle_grid
+    grid.build(s.particle_q, radius=2 * m.particle_max_radius + m.particle_cohesion)
+    deltas = torch.full((n, 3), 0.25, device=DEV)  # the kernel adds into the caller's buffer
+    N.call("nsph_xpbd_particle_contacts", grid._handle, n, N.ptr(s.particle_q), N.ptr(s.particle_qd),
+           N.ptr(m.particle_inv_mass), N.ptr(m.particle_radius), N.ptr(m.particle_flags), 0.4, 0.0,
+           m.particle_max_radius, 1e-3, 0.9, N.ptr(deltas))
+    torch.cuda.synchronize()
+    h = _host(m)
+    ref = K.xpbd_particle_contacts(_np(s.particle_q), _np(s.particle_qd), h["inv_mass"], h["radius"],
+                                   h["flags"], 0.4, 0.0, 1e-3, 0.9)
+    act = (h["flags"] & 1) != 0
+    got = _np(deltas)
+    assert np.count_nonzero(np.linalg.norm(ref, axis=1)) > n // 2
+    assert np.allclose(got[act] - 0.25, ref[act], rtol=0, atol=2e-7)
+    assert np.all(got[~act] == 0.25)
+
+
+def test_penalty_contact_kernel_matches_oracle():
+    m = _pile(seed=1)
+    n = m.particle_count
+    _print(f"semi-implicit contact kernel, {n} particles")
+    s = m.state()
+    s.particle_f.fill_(3.0)
+    m.particle_grid.build(s.particle_q, radius=2 * m.particle_max_radius)
+    N.call("nsph_eval_particle_contact", m.particle_grid._handle, n, N.ptr(s.particle_q), N.ptr(s.particle_qd),
+           N.ptr(m.particle_radius), N.ptr(m.particle_flags), m.particle_ke, m.particle_kd, m.particle_kf,
+           m.particle_mu, m.particle_cohesion, m.particle_max_radius, N.ptr(s.particle_f))
+    torch.cuda.synchronize()
+    h = _host(m)
+    ref = K.eval_particle_contact(_np(s.particle_q), _np(s.particle_qd), h["radius"], h["flags"], m.particle_ke,
+                                  m.particle_kd, m.particle_kf, m.particle_mu, m.particle_cohesion,
+                                  np.full((n, 3), 3.0))
+    scale = np.abs(ref).max()
+    assert scale > 1.0
+    assert np.allclose(_np(s.particle_f), ref, rtol=0, atol=2e-6 * scale)
+
+
+@pytest.mark.parametrize("iterations", [1, 3])
+def test_xpbd_step_matches_oracle(iterations):
+    m = _pile(n_side=10, seed=2)
+    n = m.particle_count
+    _print(f"SolverXPBD step, {n} particles, {iterations} iterations")
+    m.particle_mu = 0.3
+    solver = SolverXPBD(m, iterations=iterations)
+    s0, s1 = m.state(), m.state()
+    s0.particle_f.normal_(0.0, 0.05, generator=torch.Generator(DEV).manual_seed(4))
+    h = _host(m)
+    x0, v0, f0 = _np(s0.particle_q), _np(s0.particle_qd), _np(s0.particle_f)
+    dt = 2e-3
+    solver.step(s0, s1, None, None, dt)
+    torch.cuda.synchronize()
+    xr, vr = K.xpbd_step(x0, v0, f0, h["inv_mass"], h["radius"], h["flags"], h["world"], h["gravity"], dt,
+                         m.particle_max_velocity, 0.3, 0.0, iterations)
+    assert np.allclose(_np(s1.particle_q), xr, rtol=0, atol=2e-6)
+    assert np.allclose(_np(s1.particle_qd), vr, rtol=0, atol=2e-6 / dt)
+
+
+def test_semi_implicit_step_matches_oracle():
+    m = _pile(n_side=10, seed=5)
+    n = m.particle_count
+    _print(f"SolverSemiImplicit step, {n} particles")
+    m.set_gravity((0.0, -9.81, 0.0))
+    solver = SolverSemiImplicit(m)
+    s0, s1 = m.state(), m.state()
+    h = _host(m)
+    x0, v0 = _np(s0.particle_q), _np(s0.particle_qd)
+    dt = 1e-4
+    # unbuilt grid: no contact force (the reference kernel returns early)
+    solver.step(s0, s1, None, None, dt)
+    torch.cuda.synchronize()
+    xr, vr = K.integrate_particles(x0, v0, np.zeros_like(x0), h["inv_mass"], h["flags"], h["world"],
+                                   h["gravity"], dt, m.particle_max_velocity)
+    assert np.allclose(_np(s1.particle_qd), vr, rtol=0, atol=1e-6)
+    m.particle_grid.build(s0.particle_q, radius=2 * m.particle_max_radius)
+    solver.step(s0, s1, None, None, dt)
+    torch.cuda.synchronize()
+    xr, vr, fr = K.semi_implicit_step(x0, v0, np.zeros_like(x0), h["inv_mass"], h["radius"], h["flags"],
+                                      h["world"], h["gravity"], dt, m.particle_max_velocity, m.particle_ke,
+                                      m.particle_kd, m.particle_kf, m.particle_mu, m.particle_cohesion)
+    assert np.allclose(_np(s0.particle_f), fr, rtol=0, atol=2e-6 * np.abs(fr).max())
+    assert np.allclose(_np(s1.particle_qd), vr, rtol=0, atol=4e-6 * np.abs(vr).max())
+    assert np.allclose(_np(s1.particle_q), xr, rtol=0, atol=1e-6)
+
+
+def test_grid_size_mismatch_is_an_error():
+    m = _pile(n_side=4)
+    s = m.state()
+    m.particle_grid.build(s.particle_q[:10].contiguous(), radius=0.2)
+    d = torch.zeros_like(s.particle_qd)
+    with pytest.raises(N.NsphError):
+        N.call("nsph_xpbd_particle_contacts", m.particle_grid._handle, m.particle_count, N.ptr(s.particle_q),
+               N.ptr(s.particle_qd), N.ptr(m.particle_inv_mass), N.ptr(m.particle_radius),
+               N.ptr(m.particle_flags), 0.5, 0.0, 0.05, 1e-3, 0.9, N.ptr(d))
+
+
+def test_particle_particle_friction_uses_relative_velocity():
+    """newton/tests/test_solver_xpbd.py:20-141 with its assertions."""
+    _print("reference XPBD test: friction uses relative velocity")
+    builder = ModelBuilder(up_axis="Y")
+    r = 0.5
+    sep = 2.0 * r - 0.1
+    builder.add_particles(pos=[(0.0, 0.0, 0.0), (sep, 0.0, 0.0)], vel=[(0.0, 0.0, 10.0), (0.0, 0.0, 10.0)],
+                          mass=[1.0, 1.0], radius=[r, r])
+    model = builder.finalize(device=DEV)
+    model.set_gravity((0.0, 0.0, 0.0))
+    model.particle_mu = 1.0
+    model.particle_cohesion = 0.0
+    solver = SolverXPBD(model=model, iterations=20)
+    state0, state1 = model.state(), model.state()
+    contacts = model.contacts()
+    state0.particle_f.copy_(torch.tensor([[50.0, 0.0, 0.0], [-50.0, 0.0, 0.0]], device=DEV))
+    initial = state0.particle_qd.cpu().numpy().copy()
+    for _ in range(60):
+        model.collide(state0, contacts)
+        solver.step(state0, state1, model.control(), contacts, 1.0 / 60.0)
+        state0, state1 = state1, state0
+    final = state0.particle_qd.cpu().numpy()
+    assert round(initial[0, 2] - initial[1, 2], 5) == 0.0
+    assert round(final[0, 2] - final[1, 2], 3) == 0.0
+    assert round(final[0, 2] - final[1, 2], 3) == 0.0
+
+
+def test_particle_particle_friction_with_relative_motion():
+    """newton/tests/test_solver_xpbd.py:144-225 with its assertions."""
+    _print("reference XPBD test: friction with relative motion")
+    r = 0.5
+    sep = 2.0 * r - 0.1
+    dt = 1.0 / 30.0
+
+    def run(mu: float) -> float:
+        builder = ModelBuilder(up_axis="Y")
+        builder.add_particles(pos=[(0.0, 0.0, 0.0), (sep, 0.0, 0.0)], vel=[(0.0, 0.0, 10.0), (0.0, 0.0, 0.0)],
+                              mass=[1.0, 1.0], radius=[r, r])
+        model = builder.finalize(device=DEV)
+        model.set_gravity((0.0, 0.0, 0.0))
+        model.particle_mu = mu
+        model.particle_cohesion = 0.0
+        solver = SolverXPBD(model=model, iterations=30)
+        state0, state1 = model.state(), model.state()
+        contacts = model.contacts()
+        model.collide(state0, contacts)
+        solver.step(state0, state1, model.control(), contacts, dt)
+        q1 = state1.particle_q.cpu().numpy()
+        return float(abs(q1[0, 2] - q1[1, 2]))
+
+    slip_no_friction = run(0.0)
+    slip_with_friction = run(1.0)
+    assert slip_no_friction > 0.2
+    assert slip_with_friction < slip_no_friction * 0.95
