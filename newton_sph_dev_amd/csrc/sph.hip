@@ -102,6 +102,7 @@ struct SphDev {
     // boundary (static)
     const int2* btable;
     const float4* bxs;
+    const unsigned char* breach;  // per cell: 1 if a boundary particle lies in its 3x3x3 block
     // fluid grid
     int2* ftable;
     // sorted fluid arrays
@@ -163,6 +164,57 @@ __device__ inline bool chunk_slot_sub(const SphDev& D, int nsub, int* w, int* s)
 }
 
 // ---- build ------------------------------------------------------------------------------
+//
+// Sort keys split each cell into NSPH_XSUB slabs along x:
+//   key = (cell_off + (cz*dim_y + cy)*dim_x) * XSUB + gx,   gx = clamp(floor(XSUB f_x), 0, XSUB dim_x - 1)
+// with f_x = (x - lo_x) inv_w, the same FP32 value the cell coordinate is taken from, so
+// gx >> XSUB_BITS is the cell's cx.  Sorted order is therefore (world, cz, cy, cx, slab, id).
+// The cell table keeps one int2 per cell: x = first sorted index of the cell, y = the counts
+// of its XSUB slabs packed as bytes (slab k in bits 8k..8k+7), 0 for an empty cell.  A query
+// can then clip each row of cells to the x-extent of its sphere at slab resolution.
+
+#define NSPH_XSUB_BITS 2
+#define NSPH_XSUB (1 << NSPH_XSUB_BITS)
+#define NSPH_SLAB_MAX 255
+
+__host__ __device__ inline int slab_coord(float x, float lo, float inv_w, int dim) {
+    float f = floorf(((x - lo) * inv_w) * (float)NSPH_XSUB);
+    f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
+    const int c = (int)f;
+    const int top = dim * NSPH_XSUB - 1;
+    return c < 0 ? 0 : (c > top ? top : c);
+}
+
+__host__ __device__ inline uint32_t slab_key(const GridWorld& g, float x, float y, float z) {
+    const int gx = slab_coord(x, g.lo[0], g.inv_w, g.dim[0]);
+    const int cy = cell_coord(y, g.lo[1], g.inv_w, g.dim[1], 0);
+    const int cz = cell_coord(z, g.lo[2], g.inv_w, g.dim[2], 0);
+    return (uint32_t)(g.cell_off + (cz * g.dim[1] + cy) * g.dim[0]) * NSPH_XSUB + (uint32_t)gx;
+}
+
+// Table entry of sorted position s (keys sorted): the cell's first index and this slab's
+// count.  The last position of a slab run finds the run's start by looking back at most
+// NSPH_SLAB_MAX keys; a longer run sets bit 2 of *flag (reported as a capacity error).
+__device__ inline void mark_slab(const uint32_t* __restrict__ skeys, int s, int n,
+                                 int2* __restrict__ table, int* flag) {
+    const uint32_t key = skeys[s];
+    const uint32_t cell = key >> NSPH_XSUB_BITS;
+    if (s == 0 || (skeys[s - 1] >> NSPH_XSUB_BITS) != cell) table[cell].x = s;
+    if (s == n - 1 || skeys[s + 1] != key) {
+        const int stop = s - (NSPH_SLAB_MAX - 1) > 0 ? s - (NSPH_SLAB_MAX - 1) : 0;
+        int j = s;
+        while (j > stop && skeys[j - 1] == key) --j;
+        if (j == stop && stop > 0 && skeys[stop - 1] == key) atomicOr(flag, 2);
+        reinterpret_cast<unsigned char*>(&table[cell].y)[key & (NSPH_XSUB - 1)] = (unsigned char)(s + 1 - j);
+    }
+}
+
+// Clear the cells that the previous build occupied (its sorted keys name them).
+__global__ void k_sph_clear_prev(const uint32_t* __restrict__ prev_keys, int n, int2* __restrict__ table) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    table[prev_keys[s] >> NSPH_XSUB_BITS] = make_int2(0, 0);
+}
 
 __global__ void k_sph_keys(const float* __restrict__ q, const int* __restrict__ fid,
                            const int* __restrict__ fworld, const GridWorld* __restrict__ gw,
@@ -171,7 +223,7 @@ __global__ void k_sph_keys(const float* __restrict__ q, const int* __restrict__ 
     if (k >= n) return;
     const int id = fid[k];
     const GridWorld g = gw[fworld[k]];
-    keys[k] = cell_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+    keys[k] = slab_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
     vals[k] = (uint32_t)k;
 }
 
@@ -180,7 +232,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
                               const uint32_t* __restrict__ skeys, int n, float4* __restrict__ xs,
                               float4* __restrict__ vs, float* __restrict__ pws,
-                              int2* __restrict__ ftable) {
+                              int2* __restrict__ ftable, int* __restrict__ flags) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int k = (int)perm[s];
@@ -188,9 +240,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
     xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
     vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
     pws[s] = pprev[k];
-    const uint32_t key = skeys[s];
-    if (s == 0 || skeys[s - 1] != key) ftable[key].x = s;
-    if (s == n - 1 || skeys[s + 1] != key) ftable[key].y = s + 1;
+    mark_slab(skeys, s, n, ftable, flags);
 }
 
 __global__ void k_sph_step_init(SphDev D, int W) {
@@ -216,25 +266,44 @@ __global__ void k_sph_step_init(SphDev D, int W) {
 // four neighbour loads together, then accumulate in slot order (the order is fixed; only the
 // loads are batched).
 
-// Visit the hashed points of one row range in ascending sorted index: the cells cx0..cx0+nx-1
-// of a row are consecutive keys, so their points form the range [lo, hi).
-__device__ inline int2 row_range(const int2* __restrict__ table, int row, int x0, int nx) {
+// Number of points in the first k slabs of a cell whose packed slab counts are w.
+__device__ inline int slab_prefix(uint32_t w, int k) {
+    const uint32_t v = k >= 4 ? w : (w & ((1u << (8 * k)) - 1u));
+    return (int)((v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24));
+}
+
+// Sorted range of the points of row cells x0 .. x0+nx-1 (consecutive keys, so one range)
+// whose slab lies in [fa, fb] (global slab coordinates); empty when lo >= hi.
+__device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0, int fa, int fb) {
     int lo = 0x7fffffff, hi = 0;
     for (int ix = 0; ix < nx; ++ix) {
-        const int2 se = table[row + x0 + ix];
-        if (se.x < se.y) {
-            lo = min(lo, se.x);
-            hi = max(hi, se.y);
+        const int base = (x0 + ix) * NSPH_XSUB;
+        const int a = fa - base > 0 ? fa - base : 0;
+        const int b = fb - base < NSPH_XSUB - 1 ? fb - base : NSPH_XSUB - 1;
+        const uint32_t w = (uint32_t)c[ix].y;
+        if (a <= b && w != 0u) {
+            const int u = c[ix].x + slab_prefix(w, a), v = c[ix].x + slab_prefix(w, b + 1);
+            if (u < v) {
+                lo = min(lo, u);
+                hi = max(hi, v);
+            }
         }
     }
     return make_int2(lo, hi);
 }
 
-// Candidates j of the rows of the (clamped) query box around (qx,qy,qz), in the order z, y,
-// then ascending sorted index; calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).
+// Candidates j within r of (qx,qy,qz) from the rows of the (clamped) query box, in the order
+// z, y, then ascending sorted index; calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).
+// Each row is clipped to the sphere: with gy, gz the distances (in cells) from the query to
+// the row's y and z extent (border rows extend to infinity, as clamping puts outside points
+// there), the row is skipped when gy^2 + gz^2 >= rc^2 and otherwise only the slabs meeting
+// [f_x - hx, f_x + hx], hx = sqrt(rc^2 - gy^2 - gz^2), are visited.  Clipping drops only
+// points farther than r, so the accepted points and their order are those of the unclipped
+// rows.  rc = r/w (1 + 1/64) + 1/64 cells covers the FP32 rounding of the cell coordinates
+// (at most 2^-8 cells per axis while |x - lo| < 2^14 cell widths).
 // All row ranges are looked up first (their table loads are independent), then positions
 // are fetched RV_BATCH at a time.
-#define RV_BATCH 8
+#define RV_BATCH 4
 template <class F>
 __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table,
                                   const float4* __restrict__ xs, float qx, float qy, float qz,
@@ -244,6 +313,28 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
     axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
     axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
+    const float fx = (qx - g.lo[0]) * g.inv_w, fy = (qy - g.lo[1]) * g.inv_w, fz = (qz - g.lo[2]) * g.inv_w;
+    const float rcell = r * g.inv_w;
+    const float rc = rcell + rcell * (1.0f / 64.0f) + (1.0f / 64.0f);
+    const float rc2 = rc * rc;
+    const int xa = x0 * NSPH_XSUB, xb = (x0 + nx) * NSPH_XSUB - 1;
+    auto gap = [](float f, int c, int dim) {
+        const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
+        return fmaxf(fmaxf(e0 - f, f - e1), 0.0f);
+    };
+    auto clip = [&](int cy, int cz, int* fa, int* fb) {
+        const float gy = gap(fy, cy, g.dim[1]), gz = gap(fz, cz, g.dim[2]);
+        const float h2 = rc2 - gy * gy - gz * gz;
+        if (!(h2 > 0.0f)) return false;
+        const float hx = sqrtf(h2);
+        float a = floorf((fx - hx) * (float)NSPH_XSUB), b = floorf((fx + hx) * (float)NSPH_XSUB);
+        a = fminf(fmaxf(a, -1.0e9f), 1.0e9f);
+        b = fminf(fmaxf(b, -1.0e9f), 1.0e9f);
+        // both ends clamped into the loaded cells' slabs, as points are clamped into slabs
+        *fa = min(max((int)a, xa), xb);
+        *fb = max(min((int)b, xb), xa);
+        return *fa <= *fb;
+    };
     if (nx <= 3 && ny <= 3 && nz <= 3) {
         // common case: up to 3 x 3 rows of up to 3 cells, table loads issued together
         int2 rg[9];
@@ -251,17 +342,13 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
         for (int k = 0; k < 9; ++k) {
             const int iz = k / 3, iy = k % 3;
             rg[k] = make_int2(0x7fffffff, 0);
-            if (iz < nz && iy < ny) {
+            int fa, fb;
+            if (iz < nz && iy < ny && clip(y0 + iy, z0 + iz, &fa, &fb)) {
                 const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
                 int2 c[3];
 #pragma unroll
                 for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
-#pragma unroll
-                for (int ix = 0; ix < 3; ++ix)
-                    if (ix < nx && c[ix].x < c[ix].y) {
-                        rg[k].x = min(rg[k].x, c[ix].x);
-                        rg[k].y = max(rg[k].y, c[ix].y);
-                    }
+                rg[k] = row_slab_range(c, nx, x0, fa, fb);
             }
         }
 #pragma unroll
@@ -285,15 +372,25 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     }
     for (int iz = 0; iz < nz; ++iz) {
         for (int iy = 0; iy < ny; ++iy) {
+            int fa, fb;
+            if (!clip(y0 + iy, z0 + iz, &fa, &fb)) continue;
             const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0];
-            const int2 rg = row_range(table, row, x0, nx);
-            for (int j = rg.x; j < rg.y; j += RV_BATCH) {
+            int lo = 0x7fffffff, hi = 0;
+            for (int ix = 0; ix < nx; ++ix) {
+                const int2 c = table[row + x0 + ix];
+                const int2 rr = row_slab_range(&c, 1, x0 + ix, fa, fb);
+                if (rr.x < rr.y) {
+                    lo = min(lo, rr.x);
+                    hi = max(hi, rr.y);
+                }
+            }
+            for (int j = lo; j < hi; j += RV_BATCH) {
                 float4 p[RV_BATCH];
 #pragma unroll
-                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, rg.y - 1)];
+                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
 #pragma unroll
                 for (int u = 0; u < RV_BATCH; ++u) {
-                    if (j + u < rg.y) {
+                    if (j + u < hi) {
                         const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
                         const float d2 = (dx * dx + dy * dy) + dz * dz;
                         if (d2 < r2max) f(j + u, dx, dy, dz, d2);
@@ -350,7 +447,8 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 //            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
 // vs[s].w so that the viscosity pass fetches velocity and density together.
-__global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
+__global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_sph_nbr_density(
+    SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const GridWorld g = D.gw[w];
@@ -404,17 +502,22 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
 
     float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
     int nb = 0;
-    rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
-               [&](int, float dx, float dy, float dz, float r2) {
-                   float W, gf;
-                   kern_wg(P, r2, &W, &gf);
-                   neumaier(sum, cmp, P.psi * W);
-                   gbx += gf * dx;
-                   gby += gf * dy;
-                   gbz += gf * dz;
-                   mr2 = fminf(mr2, r2);
-                   ++nb;
-               });
+    const uint32_t own = g.cell_off + (cell_coord(xi.z, g.lo[2], g.inv_w, g.dim[2], 0) * g.dim[1] +
+                                       cell_coord(xi.y, g.lo[1], g.inv_w, g.dim[1], 0)) * g.dim[0] +
+                         cell_coord(xi.x, g.lo[0], g.inv_w, g.dim[0], 0);
+    if (D.breach[own]) {
+        rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+                   [&](int, float dx, float dy, float dz, float r2) {
+                       float W, gf;
+                       kern_wg(P, r2, &W, &gf);
+                       neumaier(sum, cmp, P.psi * W);
+                       gbx += gf * dx;
+                       gby += gf * dy;
+                       gbz += gf * dz;
+                       mr2 = fminf(mr2, r2);
+                       ++nb;
+                   });
+    }
     const float rho = sum + cmp;
     const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
     const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + xi.w * S;
@@ -786,20 +889,53 @@ __global__ void k_bnd_keys(const float* __restrict__ q, const int* __restrict__ 
     if (k >= n) return;
     const int id = bid[k];
     const GridWorld g = gw[bworld[k]];
-    keys[k] = cell_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+    keys[k] = slab_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
     vals[k] = (uint32_t)k;
 }
 
 __global__ void k_bnd_reorder(const float* __restrict__ q, const int* __restrict__ bid,
                               const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
-                              int n, float4* __restrict__ bxs, int2* __restrict__ btable) {
+                              int n, float4* __restrict__ bxs, int2* __restrict__ btable,
+                              int* __restrict__ flag) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int id = bid[perm[s]];
     bxs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], 0.f);
-    const uint32_t key = skeys[s];
-    if (s == 0 || skeys[s - 1] != key) btable[key].x = s;
-    if (s == n - 1 || skeys[s + 1] != key) btable[key].y = s + 1;
+    mark_slab(skeys, s, n, btable, flag);
+}
+
+// breach[c] = 1 when any cell of the 3x3x3 block around c (clamped to the world's box) holds
+// a boundary particle.  Every boundary candidate of a fluid particle in cell c lies in that
+// block (query ranges span at most c-1 .. c+1 per axis: the query radius is below the cell
+// width), so a zero lets the density pass skip the boundary visit.
+__global__ void k_bnd_reach(const GridWorld* __restrict__ gw, int W, int n_cells,
+                            const int2* __restrict__ btable, unsigned char* __restrict__ breach) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    int lo = 0, hi = W - 1;  // last world whose cells start at or before c
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gw[mid].cell_off <= c) lo = mid;
+        else hi = mid - 1;
+    }
+    const GridWorld g = gw[lo];
+    const int local = c - g.cell_off;
+    const int cx = local % g.dim[0], cy = (local / g.dim[0]) % g.dim[1], cz = local / (g.dim[0] * g.dim[1]);
+    unsigned char any = 0;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dim[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.dim[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.dim[0]) continue;
+                if (btable[g.cell_off + (z * g.dim[1] + y) * g.dim[0] + x].y != 0) any = 1;
+            }
+        }
+    }
+    breach[c] = any;
 }
 
 // Sorted -> particle order for the debugging / dump entry points.
@@ -920,7 +1056,7 @@ struct nsph_solver {
     // device tables
     DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
     DevBuf fid, fworld, oid;
-    DevBuf btable, bxs, ftable;
+    DevBuf btable, bxs, ftable, breach;
     DevBuf keys, vals;
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
@@ -951,6 +1087,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.chunk_world = bp<int>(S->chunk_world);
     D.btable = bp<int2>(S->btable);
     D.bxs = bp<float4>(S->bxs);
+    D.breach = bp<unsigned char>(S->breach);
     D.ftable = bp<int2>(S->ftable);
     D.xs = bp<float4>(S->xs);
     D.vs = bp<float4>(S->vs);
@@ -986,7 +1123,7 @@ static int upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
-                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs,
+                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->pr, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
@@ -1142,7 +1279,9 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
                             m->h_gravity + 3 * w);
     }
     S->total_cells = off;
-    S->key_bits = bits_for((uint32_t)(off - 1));
+    NSPH_CHECK(off * NSPH_XSUB < (size_t)1 << 32, NSPH_ERR_INVALID,
+               "nsph_solver_create: more than 2^30 cells in total");
+    S->key_bits = bits_for((uint32_t)(off * NSPH_XSUB - 1));
 
     // chunk map
     std::vector<int> fstart(W), cfirst(W), nch(W), chunk_world;
@@ -1178,6 +1317,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->btable), off + 1, make_int2(0, 0));
     hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ftable), off + 1, make_int2(0, 0));
     NSPH_TRY(buf_ensure(S->bxs, (size_t)(Nb + 1) * 16));
+    NSPH_TRY(buf_ensure(S->flags, 16));
 
     // static boundary: hash once
     if (Nb > 0) {
@@ -1191,9 +1331,20 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
             uint32_t *sk, *sp;
             rc = radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nb, S->key_bits, st, &sk, &sp);
             if (rc == NSPH_OK) {
+                hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
                 hipLaunchKernelGGL(k_bnd_reorder, dim3((Nb + 255) / 256), dim3(256), 0, st, m->d_q,
-                                   bp<int>(dbid), sp, sk, Nb, bp<float4>(S->bxs), bp<int2>(S->btable));
-                if (hipStreamSynchronize(st) != hipSuccess) rc = NSPH_ERR_HIP;
+                                   bp<int>(dbid), sp, sk, Nb, bp<float4>(S->bxs), bp<int2>(S->btable),
+                                   bp<int>(S->flags));
+                int fl = 0;
+                if (hipMemcpyAsync(&fl, S->flags.ptr, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                    hipStreamSynchronize(st) != hipSuccess) {
+                    rc = NSPH_ERR_HIP;
+                    set_error("nsph_solver_create: boundary build failed");
+                } else if (fl != 0) {
+                    rc = NSPH_ERR_CAPACITY;
+                    set_error("nsph_solver_create: more than %d boundary particles in one cell slab",
+                              NSPH_SLAB_MAX);
+                }
             }
         }
         buf_free(dbid);
@@ -1201,6 +1352,11 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         NSPH_TRY(rc);
         NSPH_HIP(hipGetLastError());
     }
+
+    NSPH_TRY(buf_ensure(S->breach, off + 1));
+    hipLaunchKernelGGL(k_bnd_reach, dim3((unsigned)((off + 255) / 256)), dim3(256), 0, st, bp<GridWorld>(S->gw), W,
+                       (int)off, bp<int2>(S->btable), bp<unsigned char>(S->breach));
+    NSPH_HIP(hipGetLastError());
 
     // the first step clears the cells named by S->skeys: point it at zero-filled keys (cell 0,
     // already empty), after the boundary build has finished with the key buffer
@@ -1315,7 +1471,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     EventPair* ep_build;
     NSPH_TRY(tic(S, KT_BUILD, &ep_build));
     // the sorted keys of the previous build (zero-filled at creation) name the cells to clear
-    hipLaunchKernelGGL(k_cells_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_sph_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
                        S->skeys, Nf, bp<int2>(S->ftable));
     hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in,
                        bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw), Nf,
@@ -1324,7 +1480,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
                         st, &S->skeys, &S->perm));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
                        S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
-                       Nf, D.xs, D.vs, D.pws, D.ftable);
+                       Nf, D.xs, D.vs, D.pws, D.ftable, D.flags);
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));

@@ -74,11 +74,20 @@ static int cell_raw(float x, float lo, float inv_w) {
 
 static int clampi(int c, int dim) { return c < 0 ? 0 : (c >= dim ? dim - 1 : c); }
 
+/* Cells are split into ORC_XSUB slabs along x for sorting: the solver's order is
+ * (world, cz, cy, cx, slab, id) with slab coordinate clamp(floor(XSUB (x - lo) inv_w)). */
+#define ORC_XSUB_BITS 2
+#define ORC_XSUB (1 << ORC_XSUB_BITS)
+
 static uint32_t key_of(const orc_gw* g, float x, float y, float z) {
-    const int cx = clampi(cell_raw(x, g->lo[0], g->inv_w), g->dim[0]);
+    float f = floorf(((x - g->lo[0]) * g->inv_w) * (float)ORC_XSUB);
+    f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);
+    int gx = (int)f;
+    const int top = g->dim[0] * ORC_XSUB - 1;
+    gx = gx < 0 ? 0 : (gx > top ? top : gx);
     const int cy = clampi(cell_raw(y, g->lo[1], g->inv_w), g->dim[1]);
     const int cz = clampi(cell_raw(z, g->lo[2], g->inv_w), g->dim[2]);
-    return (uint32_t)(g->cell_off + (cz * g->dim[1] + cy) * g->dim[0] + cx);
+    return (uint32_t)(g->cell_off + (cz * g->dim[1] + cy) * g->dim[0]) * ORC_XSUB + (uint32_t)gx;
 }
 
 static void axis_range(float x, float r, float lo, float inv_w, int dim, int* c0, int* n) {
@@ -177,8 +186,9 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
         fx[3 * s] = q_in[3 * id];
         fx[3 * s + 1] = q_in[3 * id + 1];
         fx[3 * s + 2] = q_in[3 * id + 2];
-        if (s == 0 || kv[s - 1].key != kv[s].key) ft[2 * kv[s].key] = s;
-        if (s == nf - 1 || kv[s + 1].key != kv[s].key) ft[2 * kv[s].key + 1] = s + 1;
+        const uint32_t cell = kv[s].key >> ORC_XSUB_BITS;
+        if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell] = s;
+        if (s == nf - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell + 1] = s + 1;
     }
     /* boundary (positions of the model at creation) */
     for (int k = 0; k < nb; ++k) {
@@ -193,8 +203,9 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
         L->bxs[3 * s] = pb->q0[3 * id];
         L->bxs[3 * s + 1] = pb->q0[3 * id + 1];
         L->bxs[3 * s + 2] = pb->q0[3 * id + 2];
-        if (s == 0 || kv[s - 1].key != kv[s].key) bt[2 * kv[s].key] = s;
-        if (s == nb - 1 || kv[s + 1].key != kv[s].key) bt[2 * kv[s].key + 1] = s + 1;
+        const uint32_t cell = kv[s].key >> ORC_XSUB_BITS;
+        if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) bt[2 * cell] = s;
+        if (s == nb - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) bt[2 * cell + 1] = s + 1;
     }
     free(kv);
 

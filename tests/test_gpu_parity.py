@@ -135,6 +135,53 @@ def test_step_small_dam_break_parity(dims):
     assert p_rel <= P_TOL
 
 
+@pytest.mark.parametrize("case", ["far_origin", "clamped"])
+def test_clipped_rows_keep_every_neighbour(case):
+    """Points on spheres of radius H (1 - 1e-6) and H (1 + 1e-6) around random centres, in
+    every direction, plus a random cloud: the per-row x clipping of the cell visit must not
+    lose a pair within H.  'far_origin' puts the domain origin ~2000 cells away (largest FP32
+    cell-coordinate rounding); 'clamped' uses a domain that cuts through the cloud, so points
+    outside it sit in clamped border cells.  Fluid lists vs brute force; density (which
+    includes the boundary visit) vs the unclipped FP32 restatement, bit for bit."""
+    from newton_sph_dev_amd import ModelBuilder
+
+    H = 0.02
+    rng = np.random.default_rng(17)
+    cloud = rng.uniform(0.0, 8 * H, (3000, 3))
+    centres = rng.uniform(H, 7 * H, (60, 3))
+    shells = []
+    for c in centres:
+        u = rng.normal(size=(24, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        rad = np.where(np.arange(24) % 2 == 0, H * (1 - 1e-6), H * (1 + 1e-6))
+        shells.append(c + u * rad[:, None])
+    pts = np.concatenate([cloud, centres, *shells])
+    n = len(pts)
+    mass = np.full(n, 1e-3)
+    mass[rng.random(n) < 0.3] = 0.0  # boundary samples, visited through the boundary table
+    b = ModelBuilder()
+    b.add_particles(pos=pts.tolist(), vel=[(0.0, 0.0, 0.0)] * n, mass=mass.tolist(), radius=[H / 4] * n)
+    m = b.finalize("cuda:0")
+    if case == "far_origin":
+        domain = ((-2000 * H, -3 * H, -3 * H), (10 * H, 11 * H, 11 * H))
+    else:
+        domain = ((2 * H, 1.5 * H, 2.5 * H), (6 * H, 6.5 * H, 5.5 * H))
+    _print(f"clipped rows, {case}: {n} points, 60 shells")
+    s = SolverSPH(m, H, domain=domain, max_neighbors=256, max_iters=3, min_iters=1)
+    st0, st1 = m.state(), m.state()
+    s.step(st0, st1, None, None, 1e-6)
+    torch.cuda.synchronize()
+    counts, idx = (t.cpu().numpy() for t in s.neighbors())
+    pb = O.problem_from_model(m, s)
+    fid = np.nonzero(((pb.flags & 1) != 0) & (pb.inv_mass > 0))[0]
+    bc, bi = O.neighbors_brute(pb.q0[fid], None, np.float32(H * H), cap=256)
+    brute = [tuple(sorted(int(fid[j]) for j in bi[k, : bc[k]])) for k in range(fid.size)]
+    assert _lists_as_sets(counts[fid], idx[:, fid], True) == brute
+    assert max(bc) > 20
+    r32 = O.step(pb, pb.q0, pb.qd0, precision="f32")
+    assert np.array_equal(r32["rho"], s.fields()["density"].cpu().numpy())
+
+
 def test_multistep_bitexact_and_deterministic():
     cfg = scenes.small_dam_break(10, 8, 12)
     steps = 25
