@@ -276,7 +276,9 @@ __device__ inline int slab_prefix(uint32_t w, int k) {
 // whose slab lies in [fa, fb] (global slab coordinates); empty when lo >= hi.
 __device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0, int fa, int fb) {
     int lo = 0x7fffffff, hi = 0;
-    for (int ix = 0; ix < nx; ++ix) {
+#pragma unroll
+    for (int ix = 0; ix < 3; ++ix) {
+        if (ix >= nx) break;
         const int base = (x0 + ix) * NSPH_XSUB;
         const int a = fa - base > 0 ? fa - base : 0;
         const int b = fb - base < NSPH_XSUB - 1 ? fb - base : NSPH_XSUB - 1;
@@ -401,6 +403,77 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     }
 }
 
+// rows_visit for callers that need only the accepted indices, in the same order: each row
+// range is tested in windows of 32 candidates into an accept bitmask (compare, shift, or),
+// and f(j) is then called for the set bits in ascending order.  The per-candidate work of a
+// wave no longer includes the caller's (divergent) accept code.
+template <class F>
+__device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
+                                      const float4* __restrict__ xs, float qx, float qy, float qz,
+                                      float r, float r2max, F f) {
+    const float rp = r + r * NSPH_RANGE_PAD;
+    int x0, nx, y0, ny, z0, nz;
+    axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
+    axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
+    axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
+    const float fx = (qx - g.lo[0]) * g.inv_w, fy = (qy - g.lo[1]) * g.inv_w, fz = (qz - g.lo[2]) * g.inv_w;
+    const float rcell = r * g.inv_w;
+    const float rc = rcell + rcell * (1.0f / 64.0f) + (1.0f / 64.0f);
+    const float rc2 = rc * rc;
+    const int xa = x0 * NSPH_XSUB, xb = (x0 + nx) * NSPH_XSUB - 1;
+    auto gap = [](float v, int c, int dim) {
+        const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
+        return fmaxf(fmaxf(e0 - v, v - e1), 0.0f);
+    };
+    int2 rg[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int iz = k / 3, iy = k % 3;
+        rg[k] = make_int2(0x7fffffff, 0);
+        if (iz < nz && iy < ny) {
+            const float gy = gap(fy, y0 + iy, g.dim[1]), gz = gap(fz, z0 + iz, g.dim[2]);
+            const float h2 = rc2 - gy * gy - gz * gz;
+            if (h2 > 0.0f) {
+                const float hx = sqrtf(h2);
+                float a = floorf((fx - hx) * (float)NSPH_XSUB), b = floorf((fx + hx) * (float)NSPH_XSUB);
+                a = fminf(fmaxf(a, -1.0e9f), 1.0e9f);
+                b = fminf(fmaxf(b, -1.0e9f), 1.0e9f);
+                const int fa = min(max((int)a, xa), xb), fb = max(min((int)b, xb), xa);
+                const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
+                int2 c[3];
+#pragma unroll
+                for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
+                if (fa <= fb) rg[k] = row_slab_range(c, nx, x0, fa, fb);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int hi = rg[k].y;
+        for (int base = rg[k].x; base < hi; base += 32) {
+            const int n = min(hi - base, 32);
+            uint32_t m = 0u;
+            for (int o = 0; o < n; o += RV_BATCH) {
+                float4 p[RV_BATCH];
+#pragma unroll
+                for (int v = 0; v < RV_BATCH; ++v) p[v] = xs[base + min(o + v, n - 1)];
+#pragma unroll
+                for (int v = 0; v < RV_BATCH; ++v) {
+                    const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
+                    const float d2 = (dx * dx + dy * dy) + dz * dz;
+                    m |= (d2 < r2max ? 1u : 0u) << (o + v);
+                }
+            }
+            if (n < 32) m &= (1u << n) - 1u;
+            while (m != 0u) {
+                const int b = __builtin_ctz(m);
+                m &= m - 1u;
+                f(base + b);
+            }
+        }
+    }
+}
+
 // Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
 // (B = 4 or 8 entries, one or two quads, at a time) and use(data, j) accumulates in slot
 // order.  Missing entries of the last batch repeat the first one and are not used.
@@ -447,34 +520,36 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 //            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
 // vs[s].w so that the viscosity pass fetches velocity and density together.
-__global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_sph_nbr_density(
-    SphDev D, float dt2) {
+__global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const GridWorld g = D.gw[w];
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
+    const int cx = cell_coord(xi.x, g.lo[0], g.inv_w, g.dim[0], 0);
+    const int cy = cell_coord(xi.y, g.lo[1], g.inv_w, g.dim[1], 0);
+    const int cz = cell_coord(xi.z, g.lo[2], g.inv_w, g.dim[2], 0);
     const int Nf = D.Nf, cap = D.cap;
     int4* __restrict__ out = D.nbr4 + s;
     int c = 0;
     int4 quad = make_int4(0, 0, 0, 0);
     // whole quads are stored (a partial 4-byte store per slot would write each 64-byte line
     // back several times)
-    rows_visit(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2,
-               [&](int j, float, float, float, float) {
-                   if (j == s) return;
-                   if (c < cap) {
-                       const int u = c & 3;
-                       if (u == 0) quad.x = j;
-                       else if (u == 1) quad.y = j;
-                       else if (u == 2) quad.z = j;
-                       else {
-                           quad.w = j;
-                           out[(size_t)(c >> 2) * Nf] = quad;
-                       }
-                   }
-                   ++c;
-               });
+    auto append = [&](int j) {
+        if (j == s) return;
+        if (c < cap) {
+            const int u = c & 3;
+            if (u == 0) quad.x = j;
+            else if (u == 1) quad.y = j;
+            else if (u == 2) quad.z = j;
+            else {
+                quad.w = j;
+                out[(size_t)(c >> 2) * Nf] = quad;
+            }
+        }
+        ++c;
+    };
+    rows_visit_ids(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
     {
         const int cl = c < cap ? c : cap;
         if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
@@ -502,10 +577,7 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
 
     float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
     int nb = 0;
-    const uint32_t own = g.cell_off + (cell_coord(xi.z, g.lo[2], g.inv_w, g.dim[2], 0) * g.dim[1] +
-                                       cell_coord(xi.y, g.lo[1], g.inv_w, g.dim[1], 0)) * g.dim[0] +
-                         cell_coord(xi.x, g.lo[0], g.inv_w, g.dim[0], 0);
-    if (D.breach[own]) {
+    if (D.breach[g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx]) {
         rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                    [&](int, float dx, float dy, float dz, float r2) {
                        float W, gf;
