@@ -40,15 +40,29 @@ def measured_traffic(workload: str, kernel: str):
         return None
 
 
-def algorithmic_bytes(nbar: float) -> dict[str, float]:
+def sort_passes(params) -> int:
+    """Radix passes (8-bit digits) of the cell sort: keys count the cells of every world's box
+    (cell width H (1 + 1/128), at least 3 per axis) times 4 x-slabs per cell."""
+    total = 0
+    for w in range(params.n_worlds):
+        cw = params.smoothing_length[w] * (1.0 + 1.0 / 128.0)
+        d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / cw) + 1.0, 3.0)
+        total += int(np.prod(d))
+    bits = max(int(np.ceil(np.log2(max(total * 4, 2)))), 1)
+    return (bits + 7) // 8
+
+
+def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -> dict[str, float]:
     """Compulsory bytes per fluid particle and launch, per kernel class (nbar = mean
-    fluid-fluid list length; 4 B per list entry + 4 B count)."""
+    fluid-fluid list length; 4 B per list entry + 4 B count).  `passes` radix passes of
+    20 B each; the source pass also writes iteration 0's pressure acceleration (16 B) unless
+    the MLP warm start runs between them."""
     L = 4.0 * nbar + 4.0
     return {
-        "build": 28 + 3 * 20 + 92 + 12,
+        "build": 28 + passes * 20 + 92 + 12,
         "neighbors_density": 16 + 4 + 4 + 16 + L,
         "advect": 16 + 16 + 4 + 32 + L,
-        "source": 16 + 16 + 16 + 4 + 4 + 12 + L,
+        "source": 16 + 16 + 16 + 4 + 4 + 12 + (16 if fused_source else 0) + L,
         "pressure_acc": 16 + 4 + 16 + 16 + L,
         "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 8 + L,
         "mlp_warm_start": 4 + 4 + 4 + 8 + 16 + 4 + 4 + 8,
@@ -179,7 +193,7 @@ def main():
     counts = solver.fields()["neighbor_count"]
     nf = solver.fluid_count
     nbar = float(counts.sum().item()) / max(nf, 1)
-    ab = algorithmic_bytes(nbar)
+    ab = algorithmic_bytes(nbar, sort_passes(solver.params), fused_source=args.warm_start != "mlp")
     dom = max((k for k in names if ab.get(k, 0.0) > 0), key=lambda k: per_all[k][0])
     # timed region: events only around the dominant class
     N.call("nsph_solver_set_timing", solver._handle, 1 << names.index(dom))

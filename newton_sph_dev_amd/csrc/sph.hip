@@ -642,6 +642,50 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
 // Source term and warm start:
 //   s_i = (rho0 - rho_i) - dt (sum_j m_j (v_adv_i - v_adv_j) . gradW_ij + psi v_adv_i . Gb_i)
 //   p_i = 0 | 0.5 p_prev | external;  pr_i = p_i / rho_i^2
+// Source term fused with the first pressure acceleration (zero and previous-step warm starts,
+// whose p0 every particle can form from its own inputs): one pass over the list gives s_i
+// and a^p_i of iteration 0 from pr_j = p0_j / rho_j^2, the same operations, operands and
+// order as k_sph_source followed by k_sph_pacc.
+struct XAW {
+    float4 x, a;
+    float pw, rho;
+};
+__global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
+    int w, s;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    const float4 vi = D.vadv[s];
+    const float rho = D.rho[s];
+    const bool prev = warm == NSPH_WARM_PREV;
+    const float p0 = prev ? 0.5f * D.pws[s] : 0.f;
+    const float pri = p0 / (rho * rho);
+    float sum = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    for_each_nbr<XAW, 4>(
+        D, s,
+        [&](int j) {
+            return prev ? XAW{D.xs[j], D.vadv[j], D.pws[j], D.rho[j]} : XAW{D.xs[j], D.vadv[j], 0.f, 1.f};
+        },
+        [&](const XAW& o, int) {
+            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            sum += o.x.w * dot3(vi.x - o.a.x, vi.y - o.a.y, vi.z - o.a.z, gf * dx, gf * dy, gf * dz);
+            const float prj = (prev ? 0.5f * o.pw : 0.f) / (o.rho * o.rho);
+            const float c = o.x.w * (pri + prj);
+            ax += c * (gf * dx);
+            ay += c * (gf * dy);
+            az += c * (gf * dz);
+        });
+    const float4 gb = D.gb[s];
+    sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
+    D.src[s] = (P.rho0 - rho) - dt * sum;
+    D.p[s] = p0;
+    D.pr[s] = pri;
+    const float cb = P.psi * pri;
+    D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
+}
+
 __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int warm) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -757,7 +801,16 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
     if (D.conv[w]) return;
     const int first = D.w_cfirst[w], nch = D.w_nchunks[w];
     double acc = 0.0;
-    for (int k = lane; k < nch; k += 64) acc += D.partial[first + k];
+    int k = lane;
+    // lane sums partials lane, lane+64, ... in order; eight loads are issued before the adds
+    for (; k + 7 * 64 < nch; k += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = D.partial[first + k + u * 64];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; k < nch; k += 64) acc += D.partial[first + k];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) {
         const double err = acc / ((double)D.w_nf[w] * (double)D.phys[w].rho0);
@@ -1557,7 +1610,10 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt));
-    TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
+    if (S->warm == NSPH_WARM_MLP)
+        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
+    else
+        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source_pacc, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_MLP, hipLaunchKernelGGL(k_sph_mlp_warm, dim3(C), dim3(CHUNK), 0, st, D,
                                          nsph_mlp_device_view(S->mlp), dt));
@@ -1577,7 +1633,10 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
     int launched = it0;
     if (S->Nf > 0) {
         for (int it = it0; it < it1; ++it) {
-            TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
+            // iteration 0's pressure acceleration comes from the fused source pass unless the
+            // MLP warm start set p0 after it
+            if (it > 0 || S->warm == NSPH_WARM_MLP)
+                TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
             TIMED(KT_PUPDATE, hipLaunchKernelGGL(k_sph_pupdate, dim3(C), dim3(CHUNK), 0, st, D, dt2,
                                                  (float)S->omega));
             TIMED(KT_REDUCE, hipLaunchKernelGGL(k_sph_reduce, dim3(red_blocks), dim3(256), 0, st, D,
