@@ -107,6 +107,7 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--warm-start", default="prev")
     ap.add_argument("--check-every", type=int, default=1)
+    ap.add_argument("--dt", type=float, default=0.0, help="time step (0: the workload's 1e-4 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay captured HIP graphs")
     ap.add_argument("--graph-slots", type=int, default=0,
@@ -146,7 +147,7 @@ def main():
     s0, s1 = model.state(), model.state()
     q0 = model.particle_q.clone()
     qd0 = model.particle_qd.clone()
-    dt = scenes.DT
+    dt = args.dt if args.dt > 0 else scenes.DT
     print(f"[bench] fluid={solver.fluid_count} boundary={solver.boundary_count}; warmup {args.warmup}",
           file=sys.stderr, flush=True)
     warm_iters = []

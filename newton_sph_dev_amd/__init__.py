@@ -4,9 +4,11 @@ Host code is Python; the step runs in libnsph.so (hand-written gfx950 HIP kernel
 declared in include/nsph.h) on PyTorch-ROCm tensors.
 """
 
+from . import cbor
 from .hashgrid import HashGrid
 from .sim import Axis, Contacts, Control, Model, ModelBuilder, ParticleFlags, State
 from .particle_solvers import SolverSemiImplicit, SolverXPBD
+from .recorder import RingBuffer, ViewerFile
 from .solvers import SolverBase, SolverNotifyFlags, SolverSPH
 
 __all__ = [
@@ -17,10 +19,12 @@ __all__ = [
     "Model",
     "ModelBuilder",
     "ParticleFlags",
+    "RingBuffer",
     "SolverBase",
     "SolverNotifyFlags",
     "SolverSPH",
     "SolverSemiImplicit",
     "SolverXPBD",
     "State",
+    "ViewerFile",
 ]

@@ -101,10 +101,12 @@ struct SphDev {
     const int* chunk_world;
     // boundary (static)
     const int2* btable;
+    const int* bcend;
     const float4* bxs;
     const unsigned char* breach;  // per cell: 1 if a boundary particle lies in its 3x3x3 block
     // fluid grid
     int2* ftable;
+    int* fcend;
     // sorted fluid arrays
     float4* xs;
     float4* vs;
@@ -171,11 +173,14 @@ __device__ inline bool chunk_slot_sub(const SphDev& D, int nsub, int* w, int* s)
 // gx >> XSUB_BITS is the cell's cx.  Sorted order is therefore (world, cz, cy, cx, slab, id).
 // The cell table keeps one int2 per cell: x = first sorted index of the cell, y = the counts
 // of its XSUB slabs packed as bytes (slab k in bits 8k..8k+7), 0 for an empty cell.  A query
-// can then clip each row of cells to the x-extent of its sphere at slab resolution.
+// can then clip each row of cells to the x-extent of its sphere at slab resolution.  A slab
+// of NSPH_SLAB_SAT or more points (points outside the box pile up in its clamped border
+// cells) stores the byte NSPH_SLAB_SAT; such a cell is visited whole, up to its end index
+// kept in a separate per-cell array.
 
 #define NSPH_XSUB_BITS 2
 #define NSPH_XSUB (1 << NSPH_XSUB_BITS)
-#define NSPH_SLAB_MAX 255
+#define NSPH_SLAB_SAT 255
 
 __host__ __device__ inline int slab_coord(float x, float lo, float inv_w, int dim) {
     float f = floorf(((x - lo) * inv_w) * (float)NSPH_XSUB);
@@ -192,20 +197,21 @@ __host__ __device__ inline uint32_t slab_key(const GridWorld& g, float x, float 
     return (uint32_t)(g.cell_off + (cz * g.dim[1] + cy) * g.dim[0]) * NSPH_XSUB + (uint32_t)gx;
 }
 
-// Table entry of sorted position s (keys sorted): the cell's first index and this slab's
-// count.  The last position of a slab run finds the run's start by looking back at most
-// NSPH_SLAB_MAX keys; a longer run sets bit 2 of *flag (reported as a capacity error).
+// Table entries of sorted position s (keys sorted): the cell's first index, this slab's
+// count and the cell's end.  The last position of a slab run counts the run by looking back
+// at most NSPH_SLAB_SAT - 1 keys; a run that long is stored as NSPH_SLAB_SAT (saturated).
 __device__ inline void mark_slab(const uint32_t* __restrict__ skeys, int s, int n,
-                                 int2* __restrict__ table, int* flag) {
+                                 int2* __restrict__ table, int* __restrict__ cend) {
     const uint32_t key = skeys[s];
     const uint32_t cell = key >> NSPH_XSUB_BITS;
     if (s == 0 || (skeys[s - 1] >> NSPH_XSUB_BITS) != cell) table[cell].x = s;
+    if (s == n - 1 || (skeys[s + 1] >> NSPH_XSUB_BITS) != cell) cend[cell] = s + 1;
     if (s == n - 1 || skeys[s + 1] != key) {
-        const int stop = s - (NSPH_SLAB_MAX - 1) > 0 ? s - (NSPH_SLAB_MAX - 1) : 0;
+        const int stop = s - (NSPH_SLAB_SAT - 1) > 0 ? s - (NSPH_SLAB_SAT - 1) : 0;
         int j = s;
         while (j > stop && skeys[j - 1] == key) --j;
-        if (j == stop && stop > 0 && skeys[stop - 1] == key) atomicOr(flag, 2);
-        reinterpret_cast<unsigned char*>(&table[cell].y)[key & (NSPH_XSUB - 1)] = (unsigned char)(s + 1 - j);
+        const int cnt = s + 1 - j;  // at most NSPH_SLAB_SAT, which also stands for more
+        reinterpret_cast<unsigned char*>(&table[cell].y)[key & (NSPH_XSUB - 1)] = (unsigned char)cnt;
     }
 }
 
@@ -232,7 +238,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
                               const uint32_t* __restrict__ skeys, int n, float4* __restrict__ xs,
                               float4* __restrict__ vs, float* __restrict__ pws,
-                              int2* __restrict__ ftable, int* __restrict__ flags) {
+                              int2* __restrict__ ftable, int* __restrict__ fcend) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int k = (int)perm[s];
@@ -240,7 +246,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
     xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
     vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
     pws[s] = pprev[k];
-    mark_slab(skeys, s, n, ftable, flags);
+    mark_slab(skeys, s, n, ftable, fcend);
 }
 
 __global__ void k_sph_step_init(SphDev D, int W) {
@@ -272,9 +278,16 @@ __device__ inline int slab_prefix(uint32_t w, int k) {
     return (int)((v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24));
 }
 
+__device__ inline bool slab_saturated(uint32_t w) {
+    return (w & 0xffu) == 0xffu || ((w >> 8) & 0xffu) == 0xffu || ((w >> 16) & 0xffu) == 0xffu ||
+           (w >> 24) == 0xffu;
+}
+
 // Sorted range of the points of row cells x0 .. x0+nx-1 (consecutive keys, so one range)
-// whose slab lies in [fa, fb] (global slab coordinates); empty when lo >= hi.
-__device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0, int fa, int fb) {
+// whose slab lies in [fa, fb] (global slab coordinates); empty when lo >= hi.  c[ix] is the
+// table entry of cell `cell0 + ix`; a saturated cell contributes its whole range.
+__device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0, int fa, int fb,
+                                      const int* __restrict__ cend, int cell0) {
     int lo = 0x7fffffff, hi = 0;
 #pragma unroll
     for (int ix = 0; ix < 3; ++ix) {
@@ -284,7 +297,9 @@ __device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0
         const int b = fb - base < NSPH_XSUB - 1 ? fb - base : NSPH_XSUB - 1;
         const uint32_t w = (uint32_t)c[ix].y;
         if (a <= b && w != 0u) {
-            const int u = c[ix].x + slab_prefix(w, a), v = c[ix].x + slab_prefix(w, b + 1);
+            const bool sat = slab_saturated(w);
+            const int u = sat ? c[ix].x : c[ix].x + slab_prefix(w, a);
+            const int v = sat ? cend[cell0 + ix] : c[ix].x + slab_prefix(w, b + 1);
             if (u < v) {
                 lo = min(lo, u);
                 hi = max(hi, v);
@@ -307,7 +322,7 @@ __device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0
 // are fetched RV_BATCH at a time.
 #define RV_BATCH 4
 template <class F>
-__device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table,
+__device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table, const int* __restrict__ cend,
                                   const float4* __restrict__ xs, float qx, float qy, float qz,
                                   float r, float r2max, F f) {
     const float rp = r + r * NSPH_RANGE_PAD;
@@ -350,7 +365,7 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
                 int2 c[3];
 #pragma unroll
                 for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
-                rg[k] = row_slab_range(c, nx, x0, fa, fb);
+                rg[k] = row_slab_range(c, nx, x0, fa, fb, cend, row);
             }
         }
 #pragma unroll
@@ -380,7 +395,7 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
             int lo = 0x7fffffff, hi = 0;
             for (int ix = 0; ix < nx; ++ix) {
                 const int2 c = table[row + x0 + ix];
-                const int2 rr = row_slab_range(&c, 1, x0 + ix, fa, fb);
+                const int2 rr = row_slab_range(&c, 1, x0 + ix, fa, fb, cend, row + x0 + ix);
                 if (rr.x < rr.y) {
                     lo = min(lo, rr.x);
                     hi = max(hi, rr.y);
@@ -409,6 +424,7 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
 // wave no longer includes the caller's (divergent) accept code.
 template <class F>
 __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
+                                      const int* __restrict__ cend,
                                       const float4* __restrict__ xs, float qx, float qy, float qz,
                                       float r, float r2max, F f) {
     const float rp = r + r * NSPH_RANGE_PAD;
@@ -443,7 +459,7 @@ __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict
                 int2 c[3];
 #pragma unroll
                 for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
-                if (fa <= fb) rg[k] = row_slab_range(c, nx, x0, fa, fb);
+                if (fa <= fb) rg[k] = row_slab_range(c, nx, x0, fa, fb, cend, row);
             }
         }
     }
@@ -549,7 +565,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
         }
         ++c;
     };
-    rows_visit_ids(g, D.ftable, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
+    rows_visit_ids(g, D.ftable, D.fcend, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
     {
         const int cl = c < cap ? c : cap;
         if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
@@ -578,7 +594,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
     int nb = 0;
     if (D.breach[g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx]) {
-        rows_visit(g, D.btable, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+        rows_visit(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                    [&](int, float dx, float dy, float dz, float r2) {
                        float W, gf;
                        kern_wg(P, r2, &W, &gf);
@@ -921,8 +937,13 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_features(SphDev D, float dt, 
     T[s] = punit > 0.0f ? (D.p[s] - 0.5f * D.pws[s]) / punit : 0.0f;
 }
 
-// Warm start: p0 = max(0.5 p_prev + MLP(features) p_unit, 0), written to p and pr before the
-// iterations.
+// Warm start: p0 = max(0.5 p_prev + clamp(MLP(features), +-MLP_Y_MAX) p_unit, 0), written to p
+// and pr before the iterations.  The learned correction applies only to particles with at
+// least MLP_GATE_NEIGHBOURS fluid + boundary neighbours: at the free surface and in splashes
+// -a_ii is small, p_unit large, and any network output there becomes a pressure spike that a
+// mean-error stop test does not see (such particles keep 0.5 p_prev).
+#define MLP_GATE_NEIGHBOURS 24.0f
+#define MLP_Y_MAX 8.0f
 __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, float dt) {
     __shared__ __align__(16) float xs[CHUNK * MLP_XPAD];
     __shared__ __align__(16) __bf16 h1s[CHUNK / 64][16 * MLP_H];
@@ -954,7 +975,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
     }
     __syncthreads();
     if (ok) {
-        const float p0 = fmaxf(0.5f * D.pws[s] + ys[t] * punit, 0.0f);
+        const bool bulk = f[2] * 32.0f >= MLP_GATE_NEIGHBOURS;
+        const float y = bulk ? fminf(fmaxf(ys[t], -MLP_Y_MAX), MLP_Y_MAX) : 0.0f;
+        const float p0 = fmaxf(0.5f * D.pws[s] + y * punit, 0.0f);
         const float rho = D.rho[s];
         D.p[s] = p0;
         D.pr[s] = p0 / (rho * rho);
@@ -1021,12 +1044,12 @@ __global__ void k_bnd_keys(const float* __restrict__ q, const int* __restrict__ 
 __global__ void k_bnd_reorder(const float* __restrict__ q, const int* __restrict__ bid,
                               const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys,
                               int n, float4* __restrict__ bxs, int2* __restrict__ btable,
-                              int* __restrict__ flag) {
+                              int* __restrict__ bcend) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int id = bid[perm[s]];
     bxs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], 0.f);
-    mark_slab(skeys, s, n, btable, flag);
+    mark_slab(skeys, s, n, btable, bcend);
 }
 
 // breach[c] = 1 when any cell of the 3x3x3 block around c (clamped to the world's box) holds
@@ -1181,7 +1204,7 @@ struct nsph_solver {
     // device tables
     DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
     DevBuf fid, fworld, oid;
-    DevBuf btable, bxs, ftable, breach;
+    DevBuf btable, bxs, ftable, breach, bcend, fcend;
     DevBuf keys, vals;
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
@@ -1211,6 +1234,8 @@ static SphDev make_dev(nsph_solver* S) {
     D.w_nchunks = bp<int>(S->w_nchunks);
     D.chunk_world = bp<int>(S->chunk_world);
     D.btable = bp<int2>(S->btable);
+    D.bcend = bp<int>(S->bcend);
+    D.fcend = bp<int>(S->fcend);
     D.bxs = bp<float4>(S->bxs);
     D.breach = bp<unsigned char>(S->breach);
     D.ftable = bp<int2>(S->ftable);
@@ -1248,7 +1273,7 @@ static int upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
-                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach,
+                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->pr, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
@@ -1439,6 +1464,8 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->vals, (size_t)(nmax + 1) * 4));
     NSPH_TRY(buf_ensure(S->btable, (off + 1) * sizeof(int2)));
     NSPH_TRY(buf_ensure(S->ftable, (off + 1) * sizeof(int2)));
+    NSPH_TRY(buf_ensure(S->bcend, (off + 1) * sizeof(int)));
+    NSPH_TRY(buf_ensure(S->fcend, (off + 1) * sizeof(int)));
     hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->btable), off + 1, make_int2(0, 0));
     hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ftable), off + 1, make_int2(0, 0));
     NSPH_TRY(buf_ensure(S->bxs, (size_t)(Nb + 1) * 16));
@@ -1456,19 +1483,12 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
             uint32_t *sk, *sp;
             rc = radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nb, S->key_bits, st, &sk, &sp);
             if (rc == NSPH_OK) {
-                hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
                 hipLaunchKernelGGL(k_bnd_reorder, dim3((Nb + 255) / 256), dim3(256), 0, st, m->d_q,
                                    bp<int>(dbid), sp, sk, Nb, bp<float4>(S->bxs), bp<int2>(S->btable),
-                                   bp<int>(S->flags));
-                int fl = 0;
-                if (hipMemcpyAsync(&fl, S->flags.ptr, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                    hipStreamSynchronize(st) != hipSuccess) {
+                                   bp<int>(S->bcend));
+                if (hipStreamSynchronize(st) != hipSuccess) {
                     rc = NSPH_ERR_HIP;
                     set_error("nsph_solver_create: boundary build failed");
-                } else if (fl != 0) {
-                    rc = NSPH_ERR_CAPACITY;
-                    set_error("nsph_solver_create: more than %d boundary particles in one cell slab",
-                              NSPH_SLAB_MAX);
                 }
             }
         }
@@ -1605,7 +1625,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
                         st, &S->skeys, &S->perm));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
                        S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
-                       Nf, D.xs, D.vs, D.pws, D.ftable, D.flags);
+                       Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend);
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));

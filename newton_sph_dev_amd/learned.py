@@ -6,12 +6,16 @@ MFMA kernels in libnsph: forward, gradient, Adam) predicts the correction
   density error, velocity divergence, neighbour count, previous pressure, |v| dt / H and
   distance to the nearest boundary particle
 (exact definitions in csrc/sph.hip, mlp_features).  The solver starts the Jacobi iterations
-from p0 = max(0.5 p_prev + y p_unit, 0) and always finishes with the plain Jacobi iterations to the same tolerance, so
-the plain solver stays the fallback and the judge of convergence.
+from p0 = max(0.5 p_prev + clamp(y, +-8) p_unit, 0) for particles with at least
+GATE_NEIGHBOURS fluid + boundary neighbours (0.5 p_prev elsewhere: at the free surface and
+in splashes p_unit is large and a network output would become a pressure spike), and always
+finishes with the plain Jacobi iterations to the same tolerance, so the plain solver stays
+the fallback and the judge of convergence.
 
 Training data are solver snapshots: after each step of seeded rollouts of the dam-break
-generator with random block extents, the features of every fluid particle and the target
-(converged pressure / p_unit) are exported on the GPU.
+generator with random block extents, the features and the target (converged pressure
+correction / p_unit) of every fluid particle the network is applied to are exported on the
+GPU.
 """
 
 from __future__ import annotations
@@ -30,6 +34,7 @@ from .solvers import SolverSPH
 N_PARAM = 4673
 N_FEAT = 6
 X_STRIDE = 8
+GATE_NEIGHBOURS = 24.0  # MLP_GATE_NEIGHBOURS of the warm-start kernel
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -200,8 +205,10 @@ def collect_snapshots(cfg: SnapshotConfig, device="cuda:0", keep_every: int = 1,
             a, c = c, a
             if k % keep_every == 0:
                 X, T = features(solver)
-                Xs.append(X)
-                Ts.append(T)
+                # the warm start applies the network only in the bulk (see GATE_NEIGHBOURS)
+                bulk = X[:, 2] * 32.0 >= GATE_NEIGHBOURS
+                Xs.append(X[bulk])
+                Ts.append(T[bulk])
                 snaps += cfg.worlds
     X = torch.cat(Xs)
     T = torch.cat(Ts)

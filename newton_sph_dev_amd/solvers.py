@@ -106,7 +106,7 @@ class SPHParams:
     def resolve(model: Model, smoothing_length, *, rest_density: float = 1000.0,
                 viscosity: float = 1.0e-4, boundary_volume=None, domain=None, omega: float = 0.5,
                 tolerance: float = 1.0e-3, min_iters: int = 2, max_iters: int = 100,
-                max_neighbors: int = 64, warm_start: str = "zero") -> "SPHParams":
+                max_neighbors: int = 96, warm_start: str = "zero") -> "SPHParams":
         if warm_start not in _WARM:
             raise ValueError(f"warm_start must be one of {sorted(_WARM)}")
         W = max(int(model.world_count), 1)
@@ -173,7 +173,7 @@ class SolverSPH(SolverBase):
                  rest_density: float = 1000.0, viscosity: float = 1.0e-4,
                  boundary_volume: float | Sequence[float] | None = None,
                  domain: tuple | None = None, omega: float = 0.5, tolerance: float = 1.0e-3,
-                 min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 64,
+                 min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 96,
                  warm_start: str = "zero", check_every: int = 1, report: bool = True,
                  gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None):
         super().__init__(model)

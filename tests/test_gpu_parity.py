@@ -135,14 +135,16 @@ def test_step_small_dam_break_parity(dims):
     assert p_rel <= P_TOL
 
 
-@pytest.mark.parametrize("case", ["far_origin", "clamped"])
+@pytest.mark.parametrize("case", ["far_origin", "clamped", "crowded_border"])
 def test_clipped_rows_keep_every_neighbour(case):
     """Points on spheres of radius H (1 - 1e-6) and H (1 + 1e-6) around random centres, in
     every direction, plus a random cloud: the per-row x clipping of the cell visit must not
     lose a pair within H.  'far_origin' puts the domain origin ~2000 cells away (largest FP32
     cell-coordinate rounding); 'clamped' uses a domain that cuts through the cloud, so points
-    outside it sit in clamped border cells.  Fluid lists vs brute force; density (which
-    includes the boundary visit) vs the unclipped FP32 restatement, bit for bit."""
+    outside it sit in clamped border cells; 'crowded_border' adds 300 points on a line past
+    the box's x end (1.1 H apart), all clamped into one border slab, so its count saturates
+    and the cell is visited whole.  Fluid lists vs brute force; density (which includes the
+    boundary visit) vs the unclipped FP32 restatement, bit for bit."""
     from newton_sph_dev_amd import ModelBuilder
 
     H = 0.02
@@ -156,6 +158,12 @@ def test_clipped_rows_keep_every_neighbour(case):
         rad = np.where(np.arange(24) % 2 == 0, H * (1 - 1e-6), H * (1 + 1e-6))
         shells.append(c + u * rad[:, None])
     pts = np.concatenate([cloud, centres, *shells])
+    if case == "crowded_border":
+        line = np.zeros((300, 3))
+        line[:, 0] = 8 * H + 3 * H + 1.1 * H * np.arange(300)
+        line[:, 1] = 4.2 * H
+        line[:, 2] = 3.7 * H
+        pts = np.concatenate([pts, line])
     n = len(pts)
     mass = np.full(n, 1e-3)
     mass[rng.random(n) < 0.3] = 0.0  # boundary samples, visited through the boundary table
@@ -164,8 +172,10 @@ def test_clipped_rows_keep_every_neighbour(case):
     m = b.finalize("cuda:0")
     if case == "far_origin":
         domain = ((-2000 * H, -3 * H, -3 * H), (10 * H, 11 * H, 11 * H))
-    else:
+    elif case == "clamped":
         domain = ((2 * H, 1.5 * H, 2.5 * H), (6 * H, 6.5 * H, 5.5 * H))
+    else:
+        domain = ((-2 * H, -2 * H, -2 * H), (10 * H, 10 * H, 10 * H))
     _print(f"clipped rows, {case}: {n} points, 60 shells")
     s = SolverSPH(m, H, domain=domain, max_neighbors=256, max_iters=3, min_iters=1)
     st0, st1 = m.state(), m.state()
