@@ -279,6 +279,9 @@ class ViewerFile:
     def is_paused(self) -> bool:
         return False
 
+    def should_step(self) -> bool:
+        return True
+
     def log_state(self, state) -> None:
         self.record(state)
         self._frame_count += 1
