@@ -113,6 +113,42 @@ int nsph_eval_particle_contact(nsph_grid* grid, int n, const float* d_x, const f
                                const int* d_flags, float ke, float kd, float kf, float mu, float cohesion,
                                float max_radius, float* d_f);
 
+/* ---- particle-shape soft contacts, static shapes (body = -1) ------------------------ */
+
+/* create_soft_contacts (newton/_src/geometry/kernels.py:1000-1141, launched at
+ * collide.py:1194-1230) for plane, sphere, capsule, cylinder and box shapes.  Shapes are
+ * d_shape_xform[7 s] = (px, py, pz, qx, qy, qz, qw), d_shape_scale[3 s], type (GeoType
+ * values), world and flags (ShapeFlags).  Contacts are ordered by (particle, shape);
+ * d_start[n + 1] receives each particle's first contact slot, *d_count the total (which may
+ * exceed `capacity`; slots at or beyond it are not written).  d_work: n ints of scratch. */
+int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
+                       const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
+                       const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
+                       float margin, int capacity, int* d_work, int* d_start, int* d_count, int* d_particle,
+                       int* d_shape, float* d_body_pos, float* d_body_vel, float* d_normal);
+/* solve_particle_shape_contacts (newton/_src/solvers/xpbd/kernels.py:114-222, launched at
+ * solver_xpbd.py:386-418) for static shapes: adds each active particle's corrections over
+ * its contacts into d_deltas[i]. */
+int nsph_xpbd_shape_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_v, const float* d_inv_mass,
+                             const float* d_radius, const int* d_flags, const float* d_shape_mu, float particle_mu,
+                             float particle_ka, const int* d_start, int capacity, const int* d_shape,
+                             const float* d_body_pos, const float* d_body_vel, const float* d_normal, float dt,
+                             float relaxation, float* d_deltas);
+/* eval_particle_body_contact (newton/_src/solvers/semi_implicit/kernels_contact.py:144-262,
+ * launched at kernels_contact.py:644-687) for static shapes: subtracts each active particle's
+ * contact forces from d_f[i]. */
+int nsph_eval_shape_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_v, const float* d_radius,
+                             const int* d_flags, const float* d_shape_ke, const float* d_shape_kd,
+                             const float* d_shape_kf, const float* d_shape_mu, float ke, float kd, float kf, float mu,
+                             float ka, const int* d_start, int capacity, const int* d_shape, const float* d_body_pos,
+                             const float* d_body_vel, const float* d_normal, float* d_f);
+
+/* Shape corrections after an SPH step (no reference counterpart: SolverSPH accepts contacts
+ * the way SolverXPBD does, solver_xpbd.py:386-418): particles with a nonzero correction move
+ * by it and gain correction / dt of velocity; the others are untouched. */
+int nsph_apply_shape_deltas(nsph_ctx* ctx, int n, const int* d_flags, const float* d_deltas, float dt, float v_max,
+                            float* d_x, float* d_v);
+
 /* ---- SPH solver (behind SolverBase.step, solver.py:301-316) ------------------------ */
 
 enum nsph_warm_start { NSPH_WARM_ZERO = 0, NSPH_WARM_PREV = 1, NSPH_WARM_MLP = 2 };

@@ -6,7 +6,7 @@ declared in include/nsph.h) on PyTorch-ROCm tensors.
 
 from . import cbor
 from .hashgrid import HashGrid
-from .sim import Axis, Contacts, Control, Model, ModelBuilder, ParticleFlags, State
+from .sim import Axis, Contacts, Control, GeoType, Model, ModelBuilder, ParticleFlags, ShapeFlags, State
 from .particle_solvers import SolverSemiImplicit, SolverXPBD
 from .recorder import RingBuffer, ViewerFile
 from .solvers import SolverBase, SolverNotifyFlags, SolverSPH
@@ -15,6 +15,7 @@ __all__ = [
     "Axis",
     "Contacts",
     "Control",
+    "GeoType",
     "HashGrid",
     "Model",
     "ModelBuilder",
@@ -23,6 +24,7 @@ __all__ = [
     "SolverBase",
     "SolverNotifyFlags",
     "SolverSPH",
+    "ShapeFlags",
     "SolverSemiImplicit",
     "SolverXPBD",
     "State",

@@ -130,6 +130,13 @@ SIGNATURES: dict[str, list] = {
     "nsph_xpbd_particle_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _vp],
     "nsph_apply_particle_deltas": [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _vp, _vp],
     "nsph_eval_particle_contact": [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f, _vp],
+    "nsph_soft_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp, _vp, _vp,
+                           _vp, _vp, _vp, _vp],
+    "nsph_xpbd_shape_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp, _i, _vp, _vp, _vp, _vp, _f,
+                                 _f, _vp],
+    "nsph_eval_shape_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _vp, _i, _vp,
+                                 _vp, _vp, _vp, _vp],
+    "nsph_apply_shape_deltas": [_vp, _i, _vp, _vp, _f, _f, _vp, _vp],
     "nsph_mlp_num_params": [],
     "nsph_mlp_create": [_vp, C.POINTER(_vp)],
     "nsph_mlp_destroy": [_vp],

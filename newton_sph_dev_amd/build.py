@@ -19,7 +19,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libnsph.so"
-SOURCES = ["grid.hip", "sort.hip", "sph.hip", "mlp.hip", "contacts.hip"]
+SOURCES = ["grid.hip", "sort.hip", "sph.hip", "mlp.hip", "contacts.hip", "shapes.hip"]
 ARCH = "gfx950"
 
 

@@ -30,14 +30,28 @@ def _grid_ready(grid) -> bool:
     return grid is not None and getattr(grid, "_handle", None) is not None
 
 
+def shape_contact_deltas(model: Model, contacts: Contacts, q: torch.Tensor, qd: torch.Tensor, dt: float,
+                         relaxation: float, deltas: torch.Tensor) -> None:
+    """Add the particle-shape contact corrections of xpbd/kernels.py:114-222 into deltas
+    (static shapes; particle_mu = model.soft_contact_mu, ka = model.particle_adhesion as
+    solver_xpbd.py:386-418 passes them)."""
+    N.call("nsph_xpbd_shape_contacts", _ctx(model), model.particle_count, N.ptr(q), N.ptr(qd),
+           N.ptr(model.particle_inv_mass), N.ptr(model.particle_radius), N.ptr(model.particle_flags),
+           N.ptr(model.shape_material_mu), float(model.soft_contact_mu), float(model.particle_adhesion),
+           N.ptr(contacts.soft_contact_start), contacts.soft_contact_max, N.ptr(contacts.soft_contact_shape),
+           N.ptr(contacts.soft_contact_body_pos), N.ptr(contacts.soft_contact_body_vel),
+           N.ptr(contacts.soft_contact_normal), float(dt), float(relaxation), N.ptr(deltas))
+
+
 class SolverXPBD(SolverBase):
     """Position-based particle solver (particle subset of solver_xpbd.py).
 
     One step: integrate the particles into state_out, build ``model.particle_grid`` over the
     predicted positions with radius 2 * particle_max_radius + particle_cohesion, then
-    `iterations` times: zero the deltas, project particle-particle contacts (normal + Coulomb
-    friction on the relative velocity, relaxation `soft_contact_relaxation`) and apply the
-    deltas (v = (x_new - x_init) / dt, clamped).  The position buffers ping-pong between
+    `iterations` times: zero the deltas, project particle-shape contacts (the contacts of
+    Model.collide) and particle-particle contacts (normal + Coulomb friction on the relative
+    velocity, relaxation `soft_contact_relaxation`) and apply the deltas
+    (v = (x_new - x_init) / dt, clamped).  The position buffers ping-pong between
     state_out and state_in as the reference does without gradients (solver_xpbd.py:157-201),
     so state_in's particle arrays are overwritten; the final result is in state_out.
     """
@@ -77,13 +91,26 @@ class SolverXPBD(SolverBase):
         particle_q, particle_qd = state_out.particle_q, state_out.particle_qd
         self.particle_q_init = state_in.particle_q.clone()
         particle_deltas = torch.empty_like(state_out.particle_qd)
+        if contacts is not None and model.shape_count > 0 and contacts.soft_contact_max > 0:
+            N.call("nsph_eval_shape_contacts", _ctx(model), n, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
+                   N.ptr(model.particle_radius), N.ptr(model.particle_flags), N.ptr(model.shape_material_ke),
+                   N.ptr(model.shape_material_kd), N.ptr(model.shape_material_kf), N.ptr(model.shape_material_mu),
+                   float(model.soft_contact_ke), float(model.soft_contact_kd), float(model.soft_contact_kf),
+                   float(model.soft_contact_mu), float(model.particle_adhesion), N.ptr(contacts.soft_contact_start),
+                   contacts.soft_contact_max, N.ptr(contacts.soft_contact_shape), N.ptr(contacts.soft_contact_body_pos),
+                   N.ptr(contacts.soft_contact_body_vel), N.ptr(contacts.soft_contact_normal),
+                   N.ptr(state_in.particle_f))
         self.integrate_particles(model, state_in, state_out, dt)
         grid = model.particle_grid
         use_grid = n > 1 and _grid_ready(grid)
         if use_grid:
             grid.build(state_out.particle_q, radius=model.particle_max_radius * 2.0 + model.particle_cohesion)
+        shapes = contacts is not None and model.shape_count > 0 and contacts.soft_contact_max > 0
         for _ in range(self.iterations):
             particle_deltas.zero_()
+            if shapes:
+                shape_contact_deltas(model, contacts, particle_q, particle_qd, dt, self.soft_contact_relaxation,
+                                     particle_deltas)
             if use_grid and model.particle_max_radius > 0.0:
                 N.set_stream(grid._ctx, torch.cuda.current_stream(model.device).cuda_stream)
                 N.call("nsph_xpbd_particle_contacts", grid._handle, n, N.ptr(particle_q), N.ptr(particle_qd),
@@ -101,7 +128,8 @@ class SolverSemiImplicit(SolverBase):
     """Force-based particle solver (particle subset of solver_semi_implicit.py).
 
     One step: particle-particle penalty contacts into state_in.particle_f (the reference's
-    kernel overwrites the force of every active particle), then integrate_particles.  As in
+    kernel overwrites the force of every active particle), particle-shape penalty contacts
+    subtracted from it (kernels_contact.py:144-262), then integrate_particles.  As in
     the reference, the contact forces need ``model.particle_grid`` built by the caller
     (``model.particle_grid.build(state.particle_q, radius)``); an unbuilt grid adds no
     contact force.
@@ -130,5 +158,14 @@ class SolverSemiImplicit(SolverBase):
                    N.ptr(state_in.particle_qd), N.ptr(model.particle_radius), N.ptr(model.particle_flags),
                    float(model.particle_ke), float(model.particle_kd), float(model.particle_kf),
                    float(model.particle_mu), float(model.particle_cohesion), float(model.particle_max_radius),
+                   N.ptr(state_in.particle_f))
+        if contacts is not None and model.shape_count > 0 and contacts.soft_contact_max > 0:
+            N.call("nsph_eval_shape_contacts", _ctx(model), n, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
+                   N.ptr(model.particle_radius), N.ptr(model.particle_flags), N.ptr(model.shape_material_ke),
+                   N.ptr(model.shape_material_kd), N.ptr(model.shape_material_kf), N.ptr(model.shape_material_mu),
+                   float(model.soft_contact_ke), float(model.soft_contact_kd), float(model.soft_contact_kf),
+                   float(model.soft_contact_mu), float(model.particle_adhesion), N.ptr(contacts.soft_contact_start),
+                   contacts.soft_contact_max, N.ptr(contacts.soft_contact_shape), N.ptr(contacts.soft_contact_body_pos),
+                   N.ptr(contacts.soft_contact_body_vel), N.ptr(contacts.soft_contact_normal),
                    N.ptr(state_in.particle_f))
         self.integrate_particles(model, state_in, state_out, dt)

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import enum
 from collections.abc import Sequence
+from dataclasses import dataclass, replace
 from typing import Any
 
 import numpy as np
@@ -24,6 +25,61 @@ class ParticleFlags(enum.IntEnum):
     """Flags for particle properties (newton/_src/geometry/flags.py:8-14)."""
 
     ACTIVE = 1 << 0
+
+
+class ShapeFlags(enum.IntFlag):
+    """Flags for shape properties (newton/_src/geometry/flags.py:18-33)."""
+
+    VISIBLE = 1 << 0
+    COLLIDE_SHAPES = 1 << 1
+    COLLIDE_PARTICLES = 1 << 2
+    SITE = 1 << 3
+
+
+class GeoType(enum.IntEnum):
+    """Shape geometry types (newton/_src/geometry/types.py:38-80)."""
+
+    NONE = 0
+    PLANE = 1
+    HFIELD = 2
+    SPHERE = 3
+    CAPSULE = 4
+    ELLIPSOID = 5
+    CYLINDER = 6
+    BOX = 7
+    MESH = 8
+    CONE = 9
+    CONVEX_MESH = 10
+
+
+def quat_between_vectors(a, b) -> tuple[float, float, float, float]:
+    """Unit quaternion (x, y, z, w) rotating direction a onto direction b."""
+    a = np.asarray(a, np.float64) / np.linalg.norm(a)
+    b = np.asarray(b, np.float64) / np.linalg.norm(b)
+    c = float(np.dot(a, b))
+    if c < -1.0 + 1e-12:
+        # opposite directions: half turn about any axis orthogonal to a
+        axis = np.cross(a, (1.0, 0.0, 0.0))
+        if np.linalg.norm(axis) < 1e-6:
+            axis = np.cross(a, (0.0, 1.0, 0.0))
+        axis /= np.linalg.norm(axis)
+        return (float(axis[0]), float(axis[1]), float(axis[2]), 0.0)
+    v = np.cross(a, b)
+    q = np.array([v[0], v[1], v[2], 1.0 + c])
+    q /= np.linalg.norm(q)
+    return tuple(float(x) for x in q)
+
+
+def _as_transform(xform) -> tuple[float, ...]:
+    """(px, py, pz, qx, qy, qz, qw) from None, a 7-sequence or a (position, quaternion) pair."""
+    if xform is None:
+        return (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
+    if len(xform) == 2:
+        p, q = xform
+        return tuple(float(c) for c in (*p, *q))
+    if len(xform) == 7:
+        return tuple(float(c) for c in xform)
+    raise ValueError("a transform is (position, quaternion) or 7 numbers (px, py, pz, qx, qy, qz, qw)")
 
 
 class Axis(enum.IntEnum):
@@ -69,7 +125,49 @@ def _identity_quat(q: Sequence[float]) -> bool:
 
 
 class ModelBuilder:
-    """Builds a particle Model (particle methods of newton/_src/sim/builder.py)."""
+    """Builds a particle Model (particle methods of newton/_src/sim/builder.py) with static
+    collision shapes (body = -1) for particle-shape contacts."""
+
+    @dataclass
+    class ShapeConfig:
+        """Shape material and collision settings (builder.py:231-300); the fields used by
+        particle-shape contacts, with the reference's defaults."""
+
+        density: float = 1000.0
+        ke: float = 2.5e3
+        kd: float = 100.0
+        kf: float = 1000.0
+        ka: float = 0.0
+        mu: float = 1.0
+        restitution: float = 0.0
+        margin: float = 0.0
+        gap: float | None = None
+        is_solid: bool = True
+        collision_group: int = 1
+        collision_filter_parent: bool = True
+        has_shape_collision: bool = True
+        has_particle_collision: bool = True
+        is_visible: bool = True
+        is_site: bool = False
+
+        def copy(self) -> "ModelBuilder.ShapeConfig":
+            return replace(self)
+
+        def mark_as_site(self) -> None:
+            self.is_site = True
+            self.is_visible = False
+            self.has_shape_collision = False
+            self.has_particle_collision = False
+            self.density = 0.0
+
+        @property
+        def flags(self) -> int:
+            f = 0
+            f |= int(ShapeFlags.VISIBLE) if self.is_visible else 0
+            f |= int(ShapeFlags.COLLIDE_SHAPES) if self.has_shape_collision else 0
+            f |= int(ShapeFlags.COLLIDE_PARTICLES) if self.has_particle_collision else 0
+            f |= int(ShapeFlags.SITE) if self.is_site else 0
+            return f
 
     def __init__(self, up_axis: Axis | int | str = Axis.Z, gravity: float = -9.81):
         self.up_axis = Axis.from_any(up_axis)
@@ -87,6 +185,10 @@ class ModelBuilder:
         self._flags: list[np.ndarray] = []
         self._world: list[np.ndarray] = []
         self._count = 0
+        self.default_shape_cfg = ModelBuilder.ShapeConfig()
+        self.default_site_cfg = ModelBuilder.ShapeConfig()
+        self.default_site_cfg.mark_as_site()
+        self._shapes: list[dict] = []
 
     # ---- worlds (builder.py:2815-2907) -------------------------------------------------
     @property
@@ -192,6 +294,75 @@ class ModelBuilder:
         self.add_particles(points, velocity, masses, radii, fl)
 
     # ---- finalize (builder.py:9745, 9815-9855, 10655-10668) -----------------------------
+    # ---- static shapes (builder.py:5449-5990) ----------------------------------------------
+    @property
+    def shape_count(self) -> int:
+        return len(self._shapes)
+
+    def add_shape(self, body: int, type: int, xform=None, cfg: "ModelBuilder.ShapeConfig | None" = None,
+                  scale=(1.0, 1.0, 1.0), is_static: bool = False, label: str | None = None,
+                  custom_attributes: dict | None = None, color=None) -> int:
+        if body != -1:
+            raise NotImplementedError("only static shapes (body=-1) are supported: rigid bodies are not modelled")
+        cfg = self.default_shape_cfg if cfg is None else cfg
+        self._shapes.append({
+            "xform": _as_transform(xform), "type": int(type), "scale": tuple(float(c) for c in scale),
+            "world": self._current_world, "flags": cfg.flags, "ke": cfg.ke, "kd": cfg.kd, "kf": cfg.kf,
+            "ka": cfg.ka, "mu": cfg.mu, "restitution": cfg.restitution,
+            "label": label if label is not None else f"shape_{len(self._shapes)}",
+        })
+        return len(self._shapes) - 1
+
+    def _shape_cfg(self, cfg, as_site: bool):
+        if cfg is None:
+            return self.default_site_cfg if as_site else self.default_shape_cfg
+        if as_site:
+            cfg = cfg.copy()
+            cfg.mark_as_site()
+        return cfg
+
+    def add_shape_plane(self, plane=(0.0, 0.0, 1.0, 0.0), xform=None, width: float = 10.0, length: float = 10.0,
+                        body: int = -1, cfg=None, color=None, label: str | None = None,
+                        custom_attributes: dict | None = None) -> int:
+        """Plane with normal along its local Z; from the equation a x + b y + c z + d = 0 when
+        xform is None.  width/length 0 make it infinite for collision."""
+        if xform is None:
+            n = np.asarray(plane[:3], np.float64)
+            norm = np.linalg.norm(n)
+            n = n / norm
+            pos = -(plane[3] / norm) * n
+            xform = (tuple(pos), quat_between_vectors((0.0, 0.0, 1.0), n))
+        return self.add_shape(body, GeoType.PLANE, xform, self._shape_cfg(cfg, False), (width, length, 0.0),
+                              is_static=True, label=label)
+
+    def add_ground_plane(self, height: float = 0.0, cfg=None, color=None, label: str | None = None) -> int:
+        return self.add_shape_plane(plane=(*self.up_vector, -height), width=0.0, length=0.0, cfg=cfg,
+                                    label=label or "ground_plane")
+
+    def add_shape_sphere(self, body: int, xform=None, radius: float = 1.0, cfg=None, as_site: bool = False,
+                         color=None, label: str | None = None, custom_attributes: dict | None = None) -> int:
+        return self.add_shape(body, GeoType.SPHERE, xform, self._shape_cfg(cfg, as_site), (radius, 0.0, 0.0),
+                              label=label)
+
+    def add_shape_box(self, body: int, xform=None, hx: float = 0.5, hy: float = 0.5, hz: float = 0.5, cfg=None,
+                      as_site: bool = False, color=None, label: str | None = None,
+                      custom_attributes: dict | None = None) -> int:
+        return self.add_shape(body, GeoType.BOX, xform, self._shape_cfg(cfg, as_site), (hx, hy, hz), label=label)
+
+    def add_shape_capsule(self, body: int, xform=None, radius: float = 1.0, half_height: float = 0.5, cfg=None,
+                          as_site: bool = False, color=None, label: str | None = None,
+                          custom_attributes: dict | None = None) -> int:
+        """Capsule along its local Z axis."""
+        return self.add_shape(body, GeoType.CAPSULE, xform, self._shape_cfg(cfg, as_site),
+                              (radius, half_height, 0.0), label=label)
+
+    def add_shape_cylinder(self, body: int, xform=None, radius: float = 1.0, half_height: float = 0.5, cfg=None,
+                           as_site: bool = False, color=None, label: str | None = None,
+                           custom_attributes: dict | None = None) -> int:
+        """Cylinder along its local Z axis."""
+        return self.add_shape(body, GeoType.CYLINDER, xform, self._shape_cfg(cfg, as_site),
+                              (radius, half_height, 0.0), label=label)
+
     def finalize(self, device: str | torch.device | None = None, requires_grad: bool = False) -> Model:
         if device is None:
             device = "cuda:0" if torch.cuda.is_available() else "cpu"
@@ -225,6 +396,23 @@ class ModelBuilder:
                          dtype=np.float32)
         m.gravity = torch.from_numpy(g.reshape(-1, 3)).to(device)
         m.up_axis = self.up_axis
+        S = len(self._shapes)
+        m.shape_count = S
+        if S:
+            col = lambda k, dt: torch.tensor([sh[k] for sh in self._shapes], dtype=dt, device=device)  # noqa: E731
+            m.shape_transform = col("xform", torch.float32)
+            m.shape_body = torch.full((S,), -1, dtype=torch.int32, device=device)
+            m.shape_type = col("type", torch.int32)
+            m.shape_scale = col("scale", torch.float32)
+            m.shape_world = col("world", torch.int32)
+            m.shape_flags = col("flags", torch.int32)
+            m.shape_material_ke = col("ke", torch.float32)
+            m.shape_material_kd = col("kd", torch.float32)
+            m.shape_material_kf = col("kf", torch.float32)
+            m.shape_material_ka = col("ka", torch.float32)
+            m.shape_material_mu = col("mu", torch.float32)
+            m.shape_material_restitution = col("restitution", torch.float32)
+            m.shape_label = [sh["label"] for sh in self._shapes]
         if n > 1 and m.particle_max_radius > 0.0:
             from .hashgrid import HashGrid
 
@@ -281,6 +469,25 @@ class Model:
         self.gravity = None
         self.up_axis = Axis.Z
         self.body_count = 0
+        self.shape_count = 0
+        self.shape_transform = None
+        self.shape_body = None
+        self.shape_type = None
+        self.shape_scale = None
+        self.shape_world = None
+        self.shape_flags = None
+        self.shape_material_ke = None
+        self.shape_material_kd = None
+        self.shape_material_kf = None
+        self.shape_material_ka = None
+        self.shape_material_mu = None
+        self.shape_material_restitution = None
+        self.shape_label = []
+        self.soft_contact_ke = 1.0e3
+        self.soft_contact_kd = 10.0
+        self.soft_contact_kf = 1.0e3
+        self.soft_contact_mu = 0.5
+        self.soft_contact_restitution = 0.0
 
     @property
     def particle_count(self) -> int:
@@ -297,8 +504,12 @@ class Model:
     def control(self, requires_grad: bool | None = None, clone_variables: bool = True) -> Control:
         return Control()
 
-    def contacts(self) -> Contacts:
-        return Contacts()
+    def contacts(self, soft_contact_max: int | None = None) -> Contacts:
+        """Contact buffers for collide(): soft_contact_max defaults to shape_count *
+        particle_count, as the reference's collision pipeline allocates (collide.py:814-815)."""
+        if soft_contact_max is None:
+            soft_contact_max = self.shape_count * self.particle_count
+        return Contacts(int(soft_contact_max), self.particle_count, self.device)
 
     def set_gravity(self, gravity, world: int | None = None) -> None:
         """Runtime gravity change (model.py:923-958): one vector for every world, one per
@@ -318,11 +529,30 @@ class Model:
                 raise ValueError(f"Expected {self.world_count} gravity vectors, got {len(g)}")
             self.gravity.copy_(torch.from_numpy(g.reshape(-1, 3)).to(self.gravity.device))
 
-    def collide(self, state: State, contacts: Contacts | None = None, **kwargs) -> Contacts:
-        """Contact generation (model.py:996-1030).  A particle-only model has no shapes, so
-        there are no particle-shape or rigid contacts to generate; particle-particle contacts
-        are found by the solvers through particle_grid."""
-        return contacts if contacts is not None else self.contacts()
+    def collide(self, state: State, contacts: Contacts | None = None, *, soft_contact_margin: float = 0.01,
+                **kwargs) -> Contacts:
+        """Contact generation (model.py:996-1030): particle-shape soft contacts against the
+        static shapes (collide.py:1194-1230; there are no rigid bodies, hence no rigid
+        contacts).  Particle-particle contacts are found by the solvers through particle_grid."""
+        if contacts is None:
+            contacts = self.contacts()
+        n = self.particle_count
+        if self.shape_count == 0 or n == 0 or contacts.soft_contact_max == 0:
+            contacts.clear()
+            return contacts
+        from . import _native as N
+
+        ctx = N.context(self.device.index or 0)
+        N.set_stream(ctx, torch.cuda.current_stream(self.device).cuda_stream)
+        N.call("nsph_soft_contacts", ctx, n, N.ptr(state.particle_q), N.ptr(self.particle_radius),
+               N.ptr(self.particle_flags), N.ptr(self.particle_world), self.shape_count, N.ptr(self.shape_transform),
+               N.ptr(self.shape_type), N.ptr(self.shape_scale), N.ptr(self.shape_world), N.ptr(self.shape_flags),
+               float(soft_contact_margin), contacts.soft_contact_max, N.ptr(contacts._work),
+               N.ptr(contacts.soft_contact_start), N.ptr(contacts.soft_contact_count),
+               N.ptr(contacts.soft_contact_particle), N.ptr(contacts.soft_contact_shape),
+               N.ptr(contacts.soft_contact_body_pos), N.ptr(contacts.soft_contact_body_vel),
+               N.ptr(contacts.soft_contact_normal))
+        return contacts
 
 
 class State:
@@ -355,4 +585,29 @@ class Control:
 
 
 class Contacts:
-    """newton.Contacts placeholder: the SPH step handles walls as boundary particles."""
+    """Particle-shape soft contacts (newton/_src/sim/contacts.py:90-350, soft-contact part).
+
+    Arrays of capacity soft_contact_max: soft_contact_particle, soft_contact_shape,
+    soft_contact_body_pos / _body_vel / _normal (world frame for static shapes);
+    soft_contact_count[0] = contacts found (may exceed the capacity).  Contacts are ordered by
+    (particle, shape); soft_contact_start[i] .. soft_contact_start[i + 1] are particle i's."""
+
+    def __init__(self, soft_contact_max: int = 0, particle_count: int = 0, device: torch.device | str = "cpu"):
+        dev = torch.device(device)
+        self.soft_contact_max = int(soft_contact_max)
+        self.rigid_contact_max = 0
+        m = max(self.soft_contact_max, 0)
+        self.soft_contact_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.soft_contact_particle = torch.full((m,), -1, dtype=torch.int32, device=dev)
+        self.soft_contact_shape = torch.full((m,), -1, dtype=torch.int32, device=dev)
+        self.soft_contact_body_pos = torch.zeros((m, 3), dtype=torch.float32, device=dev)
+        self.soft_contact_body_vel = torch.zeros((m, 3), dtype=torch.float32, device=dev)
+        self.soft_contact_normal = torch.zeros((m, 3), dtype=torch.float32, device=dev)
+        self.soft_contact_start = torch.zeros(particle_count + 1, dtype=torch.int32, device=dev)
+        self._work = torch.zeros(max(particle_count, 1), dtype=torch.int32, device=dev)
+
+    def clear(self) -> None:
+        self.soft_contact_count.zero_()
+        self.soft_contact_start.zero_()
+        self.soft_contact_particle.fill_(-1)
+        self.soft_contact_shape.fill_(-1)

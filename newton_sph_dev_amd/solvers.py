@@ -167,6 +167,10 @@ class SolverSPH(SolverBase):
         check_every: host convergence check interval in iterations; 0 launches max_iters
             early-exit iterations with no host read.
         report: collect a step report (one host sync per step) into `last_report`.
+
+    Walls are boundary particles.  Static collision shapes (ModelBuilder.add_shape_*) act
+    through `contacts` (from Model.collide): after the SPH step each fluid particle in contact
+    is projected out of the shapes with the XPBD particle-shape correction.
     """
 
     def __init__(self, model: Model, smoothing_length: float | Sequence[float], *,
@@ -192,6 +196,7 @@ class SolverSPH(SolverBase):
             setattr(self, name, getattr(prm, name))
         self.check_every = int(check_every)
         self.report = bool(report)
+        self.shape_relaxation = 1.0
         self.last_report: dict | None = None
         self.gather_block = int(gather_block)
         self.use_graph = bool(use_graph)
@@ -289,12 +294,28 @@ class SolverSPH(SolverBase):
                 h = self.capture(state_in, state_out, dt, self.graph_slots)
                 self._graphs[key] = h
             self.replay(h)
-            return
-        rep = N.StepReport() if self.report else None
-        N.call("nsph_solver_step", self._handle, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
-               N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(state_out.particle_f),
-               float(dt), self.check_every, C.byref(rep) if rep is not None else None)
-        self._check_report(rep)
+        else:
+            rep = N.StepReport() if self.report else None
+            N.call("nsph_solver_step", self._handle, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
+                   N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(state_out.particle_f),
+                   float(dt), self.check_every, C.byref(rep) if rep is not None else None)
+            self._check_report(rep)
+        if contacts is not None and m.shape_count > 0 and getattr(contacts, "soft_contact_max", 0) > 0:
+            self._shape_projection(state_out, contacts, dt)
+
+    def _shape_projection(self, state_out: State, contacts: Contacts, dt: float) -> None:
+        """Particle-shape contacts of Model.collide applied after the SPH step, the way
+        SolverXPBD projects them (solver_xpbd.py:386-418, xpbd/kernels.py:114-222): one
+        projection with relaxation `shape_relaxation`; particles with a correction move by it
+        and gain correction / dt of velocity."""
+        from .particle_solvers import shape_contact_deltas
+
+        m = self.model
+        deltas = torch.zeros_like(state_out.particle_q)
+        shape_contact_deltas(m, contacts, state_out.particle_q, state_out.particle_qd, dt, self.shape_relaxation,
+                             deltas)
+        N.call("nsph_apply_shape_deltas", self._ctx, m.particle_count, N.ptr(m.particle_flags), N.ptr(deltas),
+               float(dt), float(m.particle_max_velocity), N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd))
 
     def _check_report(self, rep) -> None:
         if rep is not None:
