@@ -173,10 +173,10 @@ __device__ inline bool chunk_slot_sub(const SphDev& D, int nsub, int* w, int* s)
 // gx >> XSUB_BITS is the cell's cx.  Sorted order is therefore (world, cz, cy, cx, slab, id).
 // The cell table keeps one int2 per cell: x = first sorted index of the cell, y = the counts
 // of its XSUB slabs packed as bytes (slab k in bits 8k..8k+7), 0 for an empty cell.  A query
-// can then clip each row of cells to the x-extent of its sphere at slab resolution.  A slab
+// can then clip each row of cells to the x-extent of its sphere at slab resolution.  A cell
 // of NSPH_SLAB_SAT or more points (points outside the box pile up in its clamped border
-// cells) stores the byte NSPH_SLAB_SAT; such a cell is visited whole, up to its end index
-// kept in a separate per-cell array.
+// cells) stores NSPH_SLAB_SAT as its count; such a cell is visited whole, up to its end
+// index kept in a separate per-cell array.
 
 #define NSPH_XSUB_BITS 2
 #define NSPH_XSUB (1 << NSPH_XSUB_BITS)
@@ -197,21 +197,32 @@ __host__ __device__ inline uint32_t slab_key(const GridWorld& g, float x, float 
     return (uint32_t)(g.cell_off + (cz * g.dim[1] + cy) * g.dim[0]) * NSPH_XSUB + (uint32_t)gx;
 }
 
-// Table entries of sorted position s (keys sorted): the cell's first index, this slab's
-// count and the cell's end.  The last position of a slab run counts the run by looking back
-// at most NSPH_SLAB_SAT - 1 keys; a run that long is stored as NSPH_SLAB_SAT (saturated).
-__device__ inline void mark_slab(const uint32_t* __restrict__ skeys, int s, int n,
+// Table entries of sorted position s (keys sorted): the cell's first index, its end, and
+// its slab prefix word: byte k - 1 (k = 1 .. 4) is the number of the cell's points in slabs
+// 0 .. k - 1, so byte 3 is the cell's count.  The cell's last position counts the slabs by
+// looking back at most NSPH_SLAB_SAT - 1 keys; a cell that long stores NSPH_SLAB_SAT in
+// byte 3 (saturated) and nothing else.
+__device__ inline void mark_cell(const uint32_t* __restrict__ skeys, int s, int n,
                                  int2* __restrict__ table, int* __restrict__ cend) {
     const uint32_t key = skeys[s];
     const uint32_t cell = key >> NSPH_XSUB_BITS;
     if (s == 0 || (skeys[s - 1] >> NSPH_XSUB_BITS) != cell) table[cell].x = s;
-    if (s == n - 1 || (skeys[s + 1] >> NSPH_XSUB_BITS) != cell) cend[cell] = s + 1;
-    if (s == n - 1 || skeys[s + 1] != key) {
+    if (s == n - 1 || (skeys[s + 1] >> NSPH_XSUB_BITS) != cell) {
+        cend[cell] = s + 1;
         const int stop = s - (NSPH_SLAB_SAT - 1) > 0 ? s - (NSPH_SLAB_SAT - 1) : 0;
+        uint32_t acc = 1u << (8u * (key & (NSPH_XSUB - 1)));  // per-slab counts, one byte each
         int j = s;
-        while (j > stop && skeys[j - 1] == key) --j;
+        while (j > stop && (skeys[j - 1] >> NSPH_XSUB_BITS) == cell) {
+            --j;
+            acc += 1u << (8u * (skeys[j] & (NSPH_XSUB - 1)));
+        }
         const int cnt = s + 1 - j;  // at most NSPH_SLAB_SAT, which also stands for more
-        reinterpret_cast<unsigned char*>(&table[cell].y)[key & (NSPH_XSUB - 1)] = (unsigned char)cnt;
+        uint32_t w = (uint32_t)NSPH_SLAB_SAT << 24;
+        if (cnt < NSPH_SLAB_SAT) {
+            const uint32_t p1 = acc & 0xffu, p2 = p1 + ((acc >> 8) & 0xffu), p3 = p2 + ((acc >> 16) & 0xffu);
+            w = p1 | (p2 << 8) | (p3 << 16) | ((uint32_t)cnt << 24);
+        }
+        table[cell].y = (int)w;
     }
 }
 
@@ -246,7 +257,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
     xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
     vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
     pws[s] = pprev[k];
-    mark_slab(skeys, s, n, ftable, fcend);
+    mark_cell(skeys, s, n, ftable, fcend);
 }
 
 __global__ void k_sph_step_init(SphDev D, int W) {
@@ -272,16 +283,12 @@ __global__ void k_sph_step_init(SphDev D, int W) {
 // four neighbour loads together, then accumulate in slot order (the order is fixed; only the
 // loads are batched).
 
-// Number of points in the first k slabs of a cell whose packed slab counts are w.
+// Number of points in slabs 0 .. k-1 (k = 0 .. 4) of a cell whose slab prefix word is w.
 __device__ inline int slab_prefix(uint32_t w, int k) {
-    const uint32_t v = k >= 4 ? w : (w & ((1u << (8 * k)) - 1u));
-    return (int)((v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24));
+    return k == 0 ? 0 : (int)((w >> (8 * k - 8)) & 0xffu);
 }
 
-__device__ inline bool slab_saturated(uint32_t w) {
-    return (w & 0xffu) == 0xffu || ((w >> 8) & 0xffu) == 0xffu || ((w >> 16) & 0xffu) == 0xffu ||
-           (w >> 24) == 0xffu;
-}
+__device__ inline bool slab_saturated(uint32_t w) { return (w >> 24) == (uint32_t)NSPH_SLAB_SAT; }
 
 // Sorted range of the points of row cells x0 .. x0+nx-1 (consecutive keys, so one range)
 // whose slab lies in [fa, fb] (global slab coordinates); empty when lo >= hi.  c[ix] is the
@@ -309,124 +316,23 @@ __device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0
     return make_int2(lo, hi);
 }
 
-// Candidates j within r of (qx,qy,qz) from the rows of the (clamped) query box, in the order
-// z, y, then ascending sorted index; calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).
-// Each row is clipped to the sphere: with gy, gz the distances (in cells) from the query to
-// the row's y and z extent (border rows extend to infinity, as clamping puts outside points
-// there), the row is skipped when gy^2 + gz^2 >= rc^2 and otherwise only the slabs meeting
-// [f_x - hx, f_x + hx], hx = sqrt(rc^2 - gy^2 - gz^2), are visited.  Clipping drops only
-// points farther than r, so the accepted points and their order are those of the unclipped
-// rows.  rc = r/w (1 + 1/64) + 1/64 cells covers the FP32 rounding of the cell coordinates
-// (at most 2^-8 cells per axis while |x - lo| < 2^14 cell widths).
-// All row ranges are looked up first (their table loads are independent), then positions
-// are fetched RV_BATCH at a time.
-#define RV_BATCH 4
-template <class F>
-__device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table, const int* __restrict__ cend,
-                                  const float4* __restrict__ xs, float qx, float qy, float qz,
-                                  float r, float r2max, F f) {
-    const float rp = r + r * NSPH_RANGE_PAD;
-    int x0, nx, y0, ny, z0, nz;
-    axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
-    axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
-    axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
-    const float fx = (qx - g.lo[0]) * g.inv_w, fy = (qy - g.lo[1]) * g.inv_w, fz = (qz - g.lo[2]) * g.inv_w;
-    const float rcell = r * g.inv_w;
-    const float rc = rcell + rcell * (1.0f / 64.0f) + (1.0f / 64.0f);
-    const float rc2 = rc * rc;
-    const int xa = x0 * NSPH_XSUB, xb = (x0 + nx) * NSPH_XSUB - 1;
-    auto gap = [](float f, int c, int dim) {
-        const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
-        return fmaxf(fmaxf(e0 - f, f - e1), 0.0f);
-    };
-    auto clip = [&](int cy, int cz, int* fa, int* fb) {
-        const float gy = gap(fy, cy, g.dim[1]), gz = gap(fz, cz, g.dim[2]);
-        const float h2 = rc2 - gy * gy - gz * gz;
-        if (!(h2 > 0.0f)) return false;
-        const float hx = sqrtf(h2);
-        float a = floorf((fx - hx) * (float)NSPH_XSUB), b = floorf((fx + hx) * (float)NSPH_XSUB);
-        a = fminf(fmaxf(a, -1.0e9f), 1.0e9f);
-        b = fminf(fmaxf(b, -1.0e9f), 1.0e9f);
-        // both ends clamped into the loaded cells' slabs, as points are clamped into slabs
-        *fa = min(max((int)a, xa), xb);
-        *fb = max(min((int)b, xb), xa);
-        return *fa <= *fb;
-    };
-    if (nx <= 3 && ny <= 3 && nz <= 3) {
-        // common case: up to 3 x 3 rows of up to 3 cells, table loads issued together
-        int2 rg[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int iz = k / 3, iy = k % 3;
-            rg[k] = make_int2(0x7fffffff, 0);
-            int fa, fb;
-            if (iz < nz && iy < ny && clip(y0 + iy, z0 + iz, &fa, &fb)) {
-                const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
-                int2 c[3];
-#pragma unroll
-                for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
-                rg[k] = row_slab_range(c, nx, x0, fa, fb, cend, row);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int hi = rg[k].y;
-            for (int j = rg[k].x; j < hi; j += RV_BATCH) {
-                float4 p[RV_BATCH];
-#pragma unroll
-                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
-#pragma unroll
-                for (int u = 0; u < RV_BATCH; ++u) {
-                    if (j + u < hi) {
-                        const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
-                        const float d2 = (dx * dx + dy * dy) + dz * dz;
-                        if (d2 < r2max) f(j + u, dx, dy, dz, d2);
-                    }
-                }
-            }
-        }
-        return;
-    }
-    for (int iz = 0; iz < nz; ++iz) {
-        for (int iy = 0; iy < ny; ++iy) {
-            int fa, fb;
-            if (!clip(y0 + iy, z0 + iz, &fa, &fb)) continue;
-            const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0];
-            int lo = 0x7fffffff, hi = 0;
-            for (int ix = 0; ix < nx; ++ix) {
-                const int2 c = table[row + x0 + ix];
-                const int2 rr = row_slab_range(&c, 1, x0 + ix, fa, fb, cend, row + x0 + ix);
-                if (rr.x < rr.y) {
-                    lo = min(lo, rr.x);
-                    hi = max(hi, rr.y);
-                }
-            }
-            for (int j = lo; j < hi; j += RV_BATCH) {
-                float4 p[RV_BATCH];
-#pragma unroll
-                for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
-#pragma unroll
-                for (int u = 0; u < RV_BATCH; ++u) {
-                    if (j + u < hi) {
-                        const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
-                        const float d2 = (dx * dx + dy * dy) + dz * dz;
-                        if (d2 < r2max) f(j + u, dx, dy, dz, d2);
-                    }
-                }
-            }
-        }
-    }
-}
-
-// rows_visit for callers that need only the accepted indices, in the same order: each row
-// range is tested in windows of 32 candidates into an accept bitmask (compare, shift, or),
-// and f(j) is then called for the set bits in ascending order.  The per-candidate work of a
-// wave no longer includes the caller's (divergent) accept code.
-template <class F>
-__device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
-                                      const int* __restrict__ cend,
-                                      const float4* __restrict__ xs, float qx, float qy, float qz,
-                                      float r, float r2max, F f) {
+// Rows of the query sphere, in the order z, y: scan(lo, hi) is called with the sorted range
+// [lo, hi) of each row (y0 + iy, z0 + iz) of the (clamped) query box that the sphere meets,
+// k = 3 iz + iy ascending.  r <= the cell width, so the box has at most 3 x 3 rows of at
+// most 3 cells.  Each row is clipped to the sphere: with gy, gz the distances (in cells)
+// from the query to the row's y and z extent (border rows extend to infinity, as clamping
+// puts outside points there), the row is skipped when gy^2 + gz^2 >= rc^2 and otherwise only
+// the slabs meeting [f_x - hx, f_x + hx], hx = sqrt(rc^2 - gy^2 - gz^2), are kept.  Clipping
+// drops only points farther than r, so the accepted points and their order are those of the
+// unclipped rows.  rc = r/w (1 + 1/64) + 1/64 cells covers the FP32 rounding of the cell
+// coordinates (at most 2^-8 cells per axis while |x - lo| < 2^14 cell widths).
+// The row loop stays rolled (one copy of the set-up and of the caller's scan keeps the
+// kernels small enough for the instruction cache); the table entries of row k + 1 are
+// loaded before row k is scanned, so their latency hides behind the scan.
+template <class Scan>
+__device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict__ table,
+                                     const int* __restrict__ cend, float qx, float qy, float qz,
+                                     float r, Scan scan) {
     const float rp = r + r * NSPH_RANGE_PAD;
     int x0, nx, y0, ny, z0, nz;
     axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
@@ -441,53 +347,110 @@ __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict
         const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
         return fmaxf(fmaxf(e0 - v, v - e1), 0.0f);
     };
-    int2 rg[9];
+    // clip row k to the sphere and load its table entries; false when the row is skipped
+    auto load_row = [&](int k, int2 c[3], int* fa, int* fb, int* row) {
+        const int iz = k / 3, iy = k - 3 * (k / 3);
+        if (!(iz < nz && iy < ny)) return false;
+        const float gy = gap(fy, y0 + iy, g.dim[1]), gz = gap(fz, z0 + iz, g.dim[2]);
+        const float h2 = rc2 - gy * gy - gz * gz;
+        if (!(h2 > 0.0f)) return false;
+        // the 1-ulp hardware root: the 1/64-cell pad of rc covers its error
+        const float hx = __builtin_amdgcn_sqrtf(h2);
+        float a = floorf((fx - hx) * (float)NSPH_XSUB), b = floorf((fx + hx) * (float)NSPH_XSUB);
+        a = fminf(fmaxf(a, -1.0e9f), 1.0e9f);
+        b = fminf(fmaxf(b, -1.0e9f), 1.0e9f);
+        // both ends clamped into the loaded cells' slabs, as points are clamped into slabs
+        *fa = min(max((int)a, xa), xb);
+        *fb = max(min((int)b, xb), xa);
+        if (*fa > *fb) return false;
+        *row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
 #pragma unroll
+        for (int ix = 0; ix < 3; ++ix) c[ix] = table[*row + min(ix, nx - 1)];
+        return true;
+    };
+    int2 c[3];
+    int fa = 0, fb = 0, row = 0;
+    bool ok = load_row(0, c, &fa, &fb, &row);
+#pragma unroll 1
     for (int k = 0; k < 9; ++k) {
-        const int iz = k / 3, iy = k % 3;
-        rg[k] = make_int2(0x7fffffff, 0);
-        if (iz < nz && iy < ny) {
-            const float gy = gap(fy, y0 + iy, g.dim[1]), gz = gap(fz, z0 + iz, g.dim[2]);
-            const float h2 = rc2 - gy * gy - gz * gz;
-            if (h2 > 0.0f) {
-                const float hx = sqrtf(h2);
-                float a = floorf((fx - hx) * (float)NSPH_XSUB), b = floorf((fx + hx) * (float)NSPH_XSUB);
-                a = fminf(fmaxf(a, -1.0e9f), 1.0e9f);
-                b = fminf(fmaxf(b, -1.0e9f), 1.0e9f);
-                const int fa = min(max((int)a, xa), xb), fb = max(min((int)b, xb), xa);
-                const int row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
-                int2 c[3];
+        int2 cn[3];
+        int fan = 0, fbn = 0, rown = 0;
+        const bool okn = k < 8 && load_row(k + 1, cn, &fan, &fbn, &rown);
+        if (ok) {
+            const int2 rr = row_slab_range(c, nx, x0, fa, fb, cend, row);
+            scan(rr.x, rr.y);
+        }
+        ok = okn;
+        fa = fan;
+        fb = fbn;
+        row = rown;
 #pragma unroll
-                for (int ix = 0; ix < 3; ++ix) c[ix] = table[row + min(ix, nx - 1)];
-                if (fa <= fb) rg[k] = row_slab_range(c, nx, x0, fa, fb, cend, row);
+        for (int ix = 0; ix < 3; ++ix) c[ix] = cn[ix];
+    }
+}
+
+// Candidates j within r of (qx,qy,qz) in the order z, y, then ascending sorted index;
+// calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).  Positions are fetched RV_BATCH
+// at a time.
+#define RV_BATCH 4
+template <class F>
+__device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table, const int* __restrict__ cend,
+                                  const float4* __restrict__ xs, float qx, float qy, float qz,
+                                  float r, float r2max, F f) {
+    rows_for_each(g, table, cend, qx, qy, qz, r, [&](int lo, int hi) {
+        for (int j = lo; j < hi; j += RV_BATCH) {
+            float4 p[RV_BATCH];
+#pragma unroll
+            for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
+#pragma unroll
+            for (int u = 0; u < RV_BATCH; ++u) {
+                if (j + u < hi) {
+                    const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
+                    const float d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (d2 < r2max) f(j + u, dx, dy, dz, d2);
+                }
             }
         }
-    }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int hi = rg[k].y;
-        for (int base = rg[k].x; base < hi; base += 32) {
+    });
+}
+
+// rows_visit for callers that need only the accepted indices, in the same order: each row
+// range is tested in windows of 32 candidates into an accept bitmask, and f(j) is then
+// called for the accepted candidates in ascending order.  The per-candidate work of a wave
+// no longer includes the caller's (divergent) accept code.  Candidate t of a window of n
+// enters the mask at bit n4 - 1 - t (m = 2m + accept; n4 = n rounded up to RV_BATCH), so
+// each test costs a compare and an add-with-carry; a window reads up to RV_BATCH - 1
+// positions past its row, which xs must hold (padding) and which are masked off.
+template <class F>
+__device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
+                                      const int* __restrict__ cend,
+                                      const float4* __restrict__ xs, float qx, float qy, float qz,
+                                      float r, float r2max, F f) {
+    rows_for_each(g, table, cend, qx, qy, qz, r, [&](int lo, int hi) {
+        for (int base = lo; base < hi; base += 32) {
             const int n = min(hi - base, 32);
+            const float4* __restrict__ xb = xs + base;
             uint32_t m = 0u;
-            for (int o = 0; o < n; o += RV_BATCH) {
+            int n4 = 0;
+            for (; n4 < n; n4 += RV_BATCH) {
                 float4 p[RV_BATCH];
 #pragma unroll
-                for (int v = 0; v < RV_BATCH; ++v) p[v] = xs[base + min(o + v, n - 1)];
+                for (int v = 0; v < RV_BATCH; ++v) p[v] = xb[n4 + v];
 #pragma unroll
                 for (int v = 0; v < RV_BATCH; ++v) {
                     const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
                     const float d2 = (dx * dx + dy * dy) + dz * dz;
-                    m |= (d2 < r2max ? 1u : 0u) << (o + v);
+                    m = m + m + (d2 < r2max ? 1u : 0u);
                 }
             }
-            if (n < 32) m &= (1u << n) - 1u;
+            m &= ~((1u << (n4 - n)) - 1u);  // the tail past the row
             while (m != 0u) {
-                const int b = __builtin_ctz(m);
-                m &= m - 1u;
-                f(base + b);
+                const int b = 31 - __builtin_clz(m);
+                m ^= 1u << b;
+                f(base + (n4 - 1 - b));
             }
         }
-    }
+    });
 }
 
 // Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
@@ -550,25 +513,23 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     int c = 0;
     int4 quad = make_int4(0, 0, 0, 0);
     // whole quads are stored (a partial 4-byte store per slot would write each 64-byte line
-    // back several times)
+    // back several times); entries shift into the quad from .w, so it is complete, in slot
+    // order, when every fourth entry arrives
     auto append = [&](int j) {
         if (j == s) return;
         if (c < cap) {
-            const int u = c & 3;
-            if (u == 0) quad.x = j;
-            else if (u == 1) quad.y = j;
-            else if (u == 2) quad.z = j;
-            else {
-                quad.w = j;
-                out[(size_t)(c >> 2) * Nf] = quad;
-            }
+            quad = make_int4(quad.y, quad.z, quad.w, j);
+            if ((c & 3) == 3) out[(size_t)(c >> 2) * Nf] = quad;
         }
         ++c;
     };
     rows_visit_ids(g, D.ftable, D.fcend, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
     {
         const int cl = c < cap ? c : cap;
-        if ((cl & 3) != 0) out[(size_t)(cl >> 2) * Nf] = quad;
+        const int u = cl & 3;  // entries of the last, partial quad, at its top
+        if (u == 1) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.w, 0, 0, 0);
+        if (u == 2) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.z, quad.w, 0, 0);
+        if (u == 3) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.y, quad.z, quad.w, 0);
     }
     D.ncnt[s] = c < cap ? c : cap;
     if (c > cap) atomicOr(&D.flags[0], 1);
@@ -1049,7 +1010,7 @@ __global__ void k_bnd_reorder(const float* __restrict__ q, const int* __restrict
     if (s >= n) return;
     const int id = bid[perm[s]];
     bxs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], 0.f);
-    mark_slab(skeys, s, n, btable, bcend);
+    mark_cell(skeys, s, n, btable, bcend);
 }
 
 // breach[c] = 1 when any cell of the 3x3x3 block around c (clamped to the world's box) holds
@@ -1508,7 +1469,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->keys), (size_t)(nmax + 1), 0);
     S->skeys = bp<uint32_t>(S->keys);
     const size_t nf1 = (size_t)Nf + 1;
-    NSPH_TRY(buf_ensure(S->xs, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->xs, (nf1 + RV_BATCH) * 16));  // windows read past a row's end
     NSPH_TRY(buf_ensure(S->vs, nf1 * 16));
     NSPH_TRY(buf_ensure(S->rho, nf1 * 4));
     NSPH_TRY(buf_ensure(S->aii, nf1 * 4));
