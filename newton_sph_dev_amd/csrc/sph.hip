@@ -10,9 +10,12 @@
 //
 // Layout (HBM, fluid particles in cell-sorted order s; one world = one contiguous run):
 //   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, 0)
-//   rho[s], aii[s], src[s], p[s], pr[s]=p/rho^2   float
+//   rho[s], aii[s], src[s], p[s]   float
+//   xpr[s]  float4 (x, y, z, p/rho^2)   (positions again, so that the pressure-acceleration
+//                                     gather of a world of one fluid mass loads 16 bytes)
 //   gb[s]   float4 (sum_b gradW_ib, boundary count)
-//   avisc[s], vadv[s], ap[s]  float4
+//   avisc[s], ap[s]  float4      vadv[s] float4 (v_adv, pr0), pr0 = 0.5 p_prev / rho^2 under the
+//                                previous-pressure warm start, else 0
 //   nbr4[q*Nf + s] int4  fluid-fluid neighbour list, slots 4q..4q+3 of particle s (lanes
 //                        of a wave read 64 consecutive quads)
 // Boundary particles are hashed once into their own table (bxs, btable); only the density
@@ -117,7 +120,8 @@ struct SphDev {
     float4* vadv;
     float* src;
     float* p;
-    float* pr;
+    float4* xpr;
+    int* mixed;  // per world: 1 when its fluid particles do not all have one mass
     float4* ap;
     float* pws;  // warm-start input in sorted order
     int4* nbr4;
@@ -265,6 +269,7 @@ __global__ void k_sph_step_init(SphDev D, int W) {
     if (w < W) {
         D.iters[w] = 0;
         D.err[w] = 0.0;
+        D.mixed[w] = 0;
         D.conv[w] = D.w_nf[w] == 0 ? 1 : 0;
     }
     if (w == 0) {
@@ -505,6 +510,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     const GridWorld g = D.gw[w];
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
+    if (xi.w != D.xs[D.w_fstart[w]].w) atomicOr(&D.mixed[w], 1);
     const int cx = cell_coord(xi.x, g.lo[0], g.inv_w, g.dim[0], 0);
     const int cy = cell_coord(xi.y, g.lo[1], g.inv_w, g.dim[1], 0);
     const int cz = cell_coord(xi.z, g.lo[2], g.inv_w, g.dim[2], 0);
@@ -591,7 +597,9 @@ struct XA {
 // Viscosity and advected velocity:
 //   a_visc = 10 nu sum_j (m_j/rho_j) ((v_ij . x_ij) / (r^2 + 0.01 H^2)) gradW_ij
 //   v_adv  = v + dt (g + a_visc)
-__global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
+// vadv.w carries the particle's p0 / rho^2 of the previous-pressure warm start, so that
+// the source pass fetches it with v_adv (one load per neighbour instead of three).
+__global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int warm) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
@@ -612,8 +620,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
         });
     const float avx = P.visc10 * ax, avy = P.visc10 * ay, avz = P.visc10 * az;
     D.avisc[s] = make_float4(avx, avy, avz, 0.f);
+    const float pr0 = warm == NSPH_WARM_PREV ? (0.5f * D.pws[s]) / (vi.w * vi.w) : 0.0f;
     D.vadv[s] = make_float4(vi.x + dt * (P.g[0] + avx), vi.y + dt * (P.g[1] + avy),
-                            vi.z + dt * (P.g[2] + avz), 0.f);
+                            vi.z + dt * (P.g[2] + avz), pr0);
 }
 
 // Source term and warm start:
@@ -621,12 +630,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt) {
 //   p_i = 0 | 0.5 p_prev | external;  pr_i = p_i / rho_i^2
 // Source term fused with the first pressure acceleration (zero and previous-step warm starts,
 // whose p0 every particle can form from its own inputs): one pass over the list gives s_i
-// and a^p_i of iteration 0 from pr_j = p0_j / rho_j^2, the same operations, operands and
-// order as k_sph_source followed by k_sph_pacc.
-struct XAW {
-    float4 x, a;
-    float pw, rho;
-};
+// and a^p_i of iteration 0 from pr_j = p0_j / rho_j^2 (vadv.w, formed by k_sph_advect with
+// the same operations as pri here), the same operations, operands and order as
+// k_sph_source followed by k_sph_pacc.
 __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -638,17 +644,14 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     const float p0 = prev ? 0.5f * D.pws[s] : 0.f;
     const float pri = p0 / (rho * rho);
     float sum = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XAW, 4>(
-        D, s,
-        [&](int j) {
-            return prev ? XAW{D.xs[j], D.vadv[j], D.pws[j], D.rho[j]} : XAW{D.xs[j], D.vadv[j], 0.f, 1.f};
-        },
-        [&](const XAW& o, int) {
+    for_each_nbr<XA, 4>(
+        D, s, [&](int j) { return XA{D.xs[j], D.vadv[j]}; },
+        [&](const XA& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
             const float gf = kern_gf(P, r2);
             sum += o.x.w * dot3(vi.x - o.a.x, vi.y - o.a.y, vi.z - o.a.z, gf * dx, gf * dy, gf * dz);
-            const float prj = (prev ? 0.5f * o.pw : 0.f) / (o.rho * o.rho);
+            const float prj = o.a.w;
             const float c = o.x.w * (pri + prj);
             ax += c * (gf * dx);
             ay += c * (gf * dy);
@@ -658,7 +661,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
     D.src[s] = (P.rho0 - rho) - dt * sum;
     D.p[s] = p0;
-    D.pr[s] = pri;
+    D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pri);
     const float cb = P.psi * pri;
     D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
 }
@@ -686,25 +689,35 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
     if (warm == NSPH_WARM_PREV) p0 = 0.5f * D.pws[s];
     else if (warm == NSPH_WARM_MLP) p0 = D.pws[s];
     D.p[s] = p0;
-    D.pr[s] = p0 / (rho * rho);
+    D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
 }
 
 // Pressure acceleration  a^p_i = -( sum_j m_j (pr_i + pr_j) gradW_ij + (psi pr_i) Gb_i ).
-__device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int s, float4 xi,
+// In a world whose fluid particles all have one mass, m_j = m_i and the gather loads only
+// xpr[j]; otherwise it also loads xs[j] for m_j.  Both give the same operations and values.
+__device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int w, int s, float4 xi,
                                  float* ox, float* oy, float* oz) {
-    const float pri = D.pr[s];
+    const float pri = D.xpr[s].w;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XS, 8>(
-        D, s, [&](int j) { return XS{D.xs[j], D.pr[j]}; },
-        [&](const XS& o, int) {
-            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
-            const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const float gf = kern_gf(P, r2);
-            const float c = o.x.w * (pri + o.s);
-            ax += c * (gf * dx);
-            ay += c * (gf * dy);
-            az += c * (gf * dz);
-        });
+    auto term = [&](float ox_, float oy_, float oz_, float mj, float prj) {
+        const float dx = xi.x - ox_, dy = xi.y - oy_, dz = xi.z - oz_;
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        const float gf = kern_gf(P, r2);
+        const float c = mj * (pri + prj);
+        ax += c * (gf * dx);
+        ay += c * (gf * dy);
+        az += c * (gf * dz);
+    };
+    if (__builtin_amdgcn_readfirstlane(D.mixed[w]) == 0) {
+        const float m = xi.w;
+        for_each_nbr<float4, 8>(
+            D, s, [&](int j) { return D.xpr[j]; },
+            [&](const float4& o, int) { term(o.x, o.y, o.z, m, o.w); });
+    } else {
+        for_each_nbr<XS, 8>(
+            D, s, [&](int j) { return XS{D.xs[j], D.xpr[j].w}; },
+            [&](const XS& o, int) { term(o.x.x, o.x.y, o.x.z, o.x.w, o.s); });
+    }
     const float4 gb = D.gb[s];
     const float cb = P.psi * pri;
     *ox = -(ax + cb * gb.x);
@@ -719,7 +732,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
     if (!ok) return;
     const PhysWorld P = D.phys[w];
     float ax, ay, az;
-    press_acc(D, P, s, D.xs[s], &ax, &ay, &az);
+    press_acc(D, P, w, s, D.xs[s], &ax, &ay, &az);
     D.ap[s] = make_float4(ax, ay, az, 0.f);
 }
 
@@ -757,7 +770,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
         if (a != 0.0f) pn = fmaxf(D.p[s] + omega * (si - Ap) / a, 0.0f);
         D.p[s] = pn;
         const float rho = D.rho[s];
-        D.pr[s] = pn / (rho * rho);
+        D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pn / (rho * rho));
     }
     red[t] = e;
     __syncthreads();
@@ -817,7 +830,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     float ax, ay, az;
-    press_acc(D, P, s, xi, &ax, &ay, &az);
+    press_acc(D, P, w, s, xi, &ax, &ay, &az);
     const float4 av = D.avisc[s];
     const float m = xi.w;
     const float fx = m * (av.x + ax), fy = m * (av.y + ay), fz = m * (av.z + az);
@@ -940,8 +953,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
         const float y = bulk ? fminf(fmaxf(ys[t], -MLP_Y_MAX), MLP_Y_MAX) : 0.0f;
         const float p0 = fmaxf(0.5f * D.pws[s] + y * punit, 0.0f);
         const float rho = D.rho[s];
+        const float4 xi = D.xs[s];
         D.p[s] = p0;
-        D.pr[s] = p0 / (rho * rho);
+        D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
     }
 }
 
@@ -1170,7 +1184,7 @@ struct nsph_solver {
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
-    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, pr, ap, pws, pprev, nbr, ncnt, aux;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, ap, pws, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags;
     int* h_pinned = nullptr;  // [0] act value
     int stepped = 0;
@@ -1209,7 +1223,8 @@ static SphDev make_dev(nsph_solver* S) {
     D.vadv = bp<float4>(S->vadv);
     D.src = bp<float>(S->src);
     D.p = bp<float>(S->p);
-    D.pr = bp<float>(S->pr);
+    D.xpr = bp<float4>(S->xpr);
+    D.mixed = bp<int>(S->mixed);
     D.ap = bp<float4>(S->ap);
     D.pws = bp<float>(S->pws);
     D.nbr4 = bp<int4>(S->nbr);
@@ -1236,7 +1251,7 @@ static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
                       &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
-                      &S->avisc, &S->vadv, &S->src, &S->p, &S->pr, &S->ap, &S->pws, &S->pprev,
+                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags};
     for (DevBuf* b : bufs) buf_free(*b);
@@ -1478,7 +1493,8 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->vadv, nf1 * 16));
     NSPH_TRY(buf_ensure(S->src, nf1 * 4));
     NSPH_TRY(buf_ensure(S->p, nf1 * 4));
-    NSPH_TRY(buf_ensure(S->pr, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->xpr, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->mixed, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
     NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
     NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
@@ -1590,7 +1606,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));
-    TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt));
+    TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
     else
