@@ -61,13 +61,17 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -
     return {
         "build": 28 + passes * 20 + 92 + 12,
         "neighbors_density": 16 + 4 + 4 + 16 + L,
+        # xs, vs, pws -> avisc, vadv (v_adv, p0 / rho^2)
         "advect": 16 + 16 + 4 + 32 + L,
-        "source": 16 + 16 + 16 + 4 + 4 + 12 + (16 if fused_source else 0) + L,
-        "pressure_acc": 16 + 4 + 16 + 16 + L,
-        "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 8 + L,
-        "mlp_warm_start": 4 + 4 + 4 + 8 + 16 + 4 + 4 + 8,
+        # xs, vadv, gb, rho, pws -> src, p, xpr (x, p / rho^2) [, ap]
+        "source": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 16 + (16 if fused_source else 0) + L,
+        # xs, xpr, gb -> ap
+        "pressure_acc": 16 + 16 + 16 + 16 + L,
+        # xs, ap, gb, src, aii, p, rho -> p, xpr
+        "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + 16 + L,
+        "mlp_warm_start": 4 + 4 + 4 + 8 + 16 + 4 + 4 + 16 + 16 + 4,
         "reduce": 0.0,
-        "final": 16 + 4 + 16 + 16 + 16 + 12 + 4 + 40 + L,
+        "final": 16 + 16 + 16 + 16 + 16 + 12 + 4 + 40 + L,
         "passthrough": 0.0,
     }
 
