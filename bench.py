@@ -41,15 +41,16 @@ def measured_traffic(workload: str, kernel: str):
 
 
 def sort_passes(params) -> int:
-    """Radix passes (8-bit digits) of the cell sort: keys count the cells of every world's box
-    (cell width H (1 + 1/128), at least 3 per axis) times 4 x-slabs per cell."""
+    """Radix passes of the cell sort (8-bit digits, or 9-bit where that saves a pass): keys
+    count the cells of every world's box (cell width H (1 + 1/128), at least 3 per axis)
+    times 4 x-slabs per cell."""
     total = 0
     for w in range(params.n_worlds):
         cw = params.smoothing_length[w] * (1.0 + 1.0 / 128.0)
         d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / cw) + 1.0, 3.0)
         total += int(np.prod(d))
     bits = max(int(np.ceil(np.log2(max(total * 4, 2)))), 1)
-    return (bits + 7) // 8
+    return min((bits + 7) // 8, (bits + 8) // 9)
 
 
 def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -> dict[str, float]:

@@ -1,7 +1,8 @@
-// Stable LSD radix sort of (uint32 key, uint32 value) pairs, 8-bit digits, for gfx950.
+// Stable LSD radix sort of (uint32 key, uint32 value) pairs for gfx950, with 8- or 9-bit
+// digits (9 when that saves a pass: keys of 25-27 bits sort in 3 passes instead of 4).
 //
-// Per pass (digit d = (key >> shift) & 255):
-//   k_rs_hist    : one workgroup per tile of RS_TILE keys builds a 256-bin LDS histogram
+// Per pass (digit d = (key >> shift) & (NB - 1), NB = 2^digit bits):
+//   k_rs_hist    : one workgroup per tile of RS_TILE keys builds an NB-bin LDS histogram
 //                  and stores it digit-major: hist[d * n_tiles + tile].
 //   k_rs_scan    : one workgroup per digit turns its row into an exclusive prefix over
 //                  tiles and stores the digit total.
@@ -46,21 +47,44 @@ __device__ inline uint32_t block_excl_scan_256(uint32_t v, uint32_t* tmp, uint32
     return wave_base + x - v;
 }
 
+// Exclusive scan over NB bins (NB / 256 consecutive bins per thread): out[b] = sum in[< b].
+// in and out may be the same array; get(b) reads bin b.
+template <int NB, class Get, class Put>
+__device__ inline void bins_excl_scan(Get get, Put put, uint32_t* tmp) {
+    constexpr int PER = NB / RS_BLOCK;
+    const int t = threadIdx.x;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        v[i] = get(t * PER + i);
+        sum += v[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan_256(sum, tmp, &tot);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        put(t * PER + i, run);
+        run += v[i];
+    }
+}
+
+template <int DB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict__ keys, int n,
                                                       int shift, uint32_t* __restrict__ hist,
                                                       int n_tiles) {
-    __shared__ uint32_t h[256];
+    constexpr int NB = 1 << DB;
+    __shared__ uint32_t h[NB];
     const int t = threadIdx.x;
-    h[t] = 0;
+    for (int b = t; b < NB; b += RS_BLOCK) h[b] = 0;
     __syncthreads();
     const int base = blockIdx.x * RS_TILE;
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
         int idx = base + i * RS_BLOCK + t;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (uint32_t)(NB - 1)], 1u);
     }
     __syncthreads();
-    hist[(size_t)t * n_tiles + blockIdx.x] = h[t];
+    for (int b = t; b < NB; b += RS_BLOCK) hist[(size_t)b * n_tiles + blockIdx.x] = h[b];
 }
 
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scan(uint32_t* __restrict__ hist, int n_tiles,
@@ -79,14 +103,17 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scan(uint32_t* __restrict__ his
     if (threadIdx.x == 0) digit_tot[blockIdx.x] = carry;
 }
 
+template <int DB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n, int shift,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot, int n_tiles) {
-    __shared__ uint32_t s_gbase[256];   // global start of this tile's run of digit d
-    __shared__ uint32_t s_lstart[256];  // start of digit d inside the regrouped tile
-    __shared__ uint32_t s_run[256];     // items of digit d ranked so far
-    __shared__ uint32_t s_wcnt[RS_WAVES][256];
+    constexpr int NB = 1 << DB;
+    constexpr uint32_t MASK = (uint32_t)(NB - 1);
+    __shared__ uint32_t s_gbase[NB];   // global start of this tile's run of digit d
+    __shared__ uint32_t s_lstart[NB];  // start of digit d inside the regrouped tile
+    __shared__ uint32_t s_run[NB];     // items of digit d ranked so far
+    __shared__ uint32_t s_wcnt[RS_WAVES][NB];
     __shared__ uint32_t s_keys[RS_TILE];
     __shared__ uint32_t s_vals[RS_TILE];
     __shared__ uint32_t tmp[RS_WAVES];
@@ -95,21 +122,21 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
     const int base = blockIdx.x * RS_TILE;
     const int n_valid = min(RS_TILE, n - base);
 
-    {
-        uint32_t tot;
-        uint32_t dpre = block_excl_scan_256(digit_tot[t], tmp, &tot);
-        s_gbase[t] = dpre + hist[(size_t)t * n_tiles + blockIdx.x];
-    }
-    s_run[t] = 0;
+    bins_excl_scan<NB>([&](int b) { return digit_tot[b]; },
+                       [&](int b, uint32_t pre) { s_gbase[b] = pre + hist[(size_t)b * n_tiles + blockIdx.x]; },
+                       tmp);
+    for (int b = t; b < NB; b += RS_BLOCK) {
+        s_run[b] = 0;
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][t] = 0;
+        for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][b] = 0;
+    }
 
     uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
         int idx = base + i * RS_BLOCK + t;
         bool ok = idx < n;
-        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;  // invalid items rank last in digit 255
+        key[i] = ok ? kin[idx] : 0xFFFFFFFFu;  // invalid items rank last in the top digit
         val[i] = ok ? vin[idx] : 0u;
     }
     __syncthreads();
@@ -117,10 +144,10 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
-        const uint32_t d = (key[i] >> shift) & 255u;
+        const uint32_t d = (key[i] >> shift) & MASK;
         uint64_t peers = ~0ull;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < DB; ++b) {
             const int bit = (d >> b) & 1;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -132,24 +159,23 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
         for (int w = 0; w < wave; ++w) r += s_wcnt[w][d];
         rank[i] = r;
         __syncthreads();
-        uint32_t add = 0;
+        for (int b = t; b < NB; b += RS_BLOCK) {
+            uint32_t add = 0;
 #pragma unroll
-        for (int w = 0; w < RS_WAVES; ++w) {
-            add += s_wcnt[w][t];
-            s_wcnt[w][t] = 0;
+            for (int w = 0; w < RS_WAVES; ++w) {
+                add += s_wcnt[w][b];
+                s_wcnt[w][b] = 0;
+            }
+            s_run[b] += add;
         }
-        s_run[t] += add;
         __syncthreads();
     }
 
-    {
-        uint32_t tot;
-        s_lstart[t] = block_excl_scan_256(s_run[t], tmp, &tot);
-    }
+    bins_excl_scan<NB>([&](int b) { return s_run[b]; }, [&](int b, uint32_t pre) { s_lstart[b] = pre; }, tmp);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; ++i) {
-        const uint32_t d = (key[i] >> shift) & 255u;
+        const uint32_t d = (key[i] >> shift) & MASK;
         const uint32_t pos = s_lstart[d] + rank[i];
         s_keys[pos] = key[i];
         s_vals[pos] = val[i];
@@ -160,7 +186,7 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
         const int p = i * RS_BLOCK + t;
         if (p < n_valid) {
             const uint32_t k = s_keys[p];
-            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t d = (k >> shift) & MASK;
             const uint32_t g = s_gbase[d] + (uint32_t)p - s_lstart[d];
             kout[g] = k;
             vout[g] = s_vals[p];
@@ -173,8 +199,8 @@ int radix_reserve(RadixSort& rs, int n) {
     const int n_tiles = (n + RS_TILE - 1) / RS_TILE;
     NSPH_TRY(buf_ensure(rs.keys_alt, (size_t)n * 4));
     NSPH_TRY(buf_ensure(rs.vals_alt, (size_t)n * 4));
-    NSPH_TRY(buf_ensure(rs.hist, (size_t)n_tiles * 256 * 4));
-    NSPH_TRY(buf_ensure(rs.digit_tot, 256 * 4));
+    NSPH_TRY(buf_ensure(rs.hist, (size_t)n_tiles * 512 * 4));
+    NSPH_TRY(buf_ensure(rs.digit_tot, 512 * 4));
     rs.capacity = n;
     return NSPH_OK;
 }
@@ -190,16 +216,26 @@ int radix_sort(RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits,
         return NSPH_OK;
     }
     const int n_tiles = (n + RS_TILE - 1) / RS_TILE;
-    const int passes = bits <= 0 ? 0 : (bits + 7) / 8;
+    const int p8 = bits <= 0 ? 0 : (bits + 7) / 8, p9 = bits <= 0 ? 0 : (bits + 8) / 9;
+    const int db = p9 < p8 ? 9 : 8;  // 9-bit digits when they save a pass
+    const int passes = db == 9 ? p9 : p8;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_rs_hist, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
-                           bp<uint32_t>(rs.hist), n_tiles);
-        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_BLOCK), 0, stream,
-                           bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot));
-        hipLaunchKernelGGL(k_rs_scatter, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
-                           vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot),
-                           n_tiles);
+        const int shift = db * p;
+        if (db == 9) {
+            hipLaunchKernelGGL(k_rs_hist<9>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
+                               bp<uint32_t>(rs.hist), n_tiles);
+            hipLaunchKernelGGL(k_rs_scan, dim3(512), dim3(RS_BLOCK), 0, stream,
+                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot));
+            hipLaunchKernelGGL(k_rs_scatter<9>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
+                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles);
+        } else {
+            hipLaunchKernelGGL(k_rs_hist<8>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
+                               bp<uint32_t>(rs.hist), n_tiles);
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_BLOCK), 0, stream,
+                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot));
+            hipLaunchKernelGGL(k_rs_scatter<8>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
+                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles);
+        }
         NSPH_HIP(hipGetLastError());
         uint32_t* t0 = ka; ka = kb; kb = t0;
         uint32_t* t1 = va; va = vb; vb = t1;
