@@ -60,7 +60,7 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -
     the MLP warm start runs between them."""
     L = 4.0 * nbar + 4.0
     return {
-        "build": 28 + passes * 20 + 92 + 12,
+        "build": 28 + passes * 20 + 92 + 16,
         "neighbors_density": 16 + 4 + 4 + 16 + L,
         # xs, vs, pws -> avisc, vadv (v_adv, p0 / rho^2)
         "advect": 16 + 16 + 4 + 32 + L,
@@ -72,8 +72,11 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -
         "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + 16 + L,
         "mlp_warm_start": 4 + 4 + 4 + 8 + 16 + 4 + 4 + 16 + 16 + 4,
         "reduce": 0.0,
-        "final": 16 + 16 + 16 + 16 + 16 + 12 + 4 + 40 + L,
+        # xs, xpr, gb, avisc, p -> fp (f, p)
+        "final": 16 + 16 + 16 + 16 + 4 + 16 + L,
         "passthrough": 0.0,
+        # inv, fid, fworld, fp, xs, vs, inv_mass -> q, qd, f, pprev (particle order)
+        "integrate": 4 + 4 + 4 + 16 + 16 + 16 + 4 + 40,
     }
 
 

@@ -121,6 +121,7 @@ struct SphDev {
     float* src;
     float* p;
     float4* xpr;
+    float4* fp;  // (force, p) of the step's end, sorted order
     int* mixed;  // per world: 1 when its fluid particles do not all have one mass
     float4* ap;
     float* pws;  // warm-start input in sorted order
@@ -253,11 +254,13 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
                               const uint32_t* __restrict__ skeys, int n, float4* __restrict__ xs,
                               float4* __restrict__ vs, float* __restrict__ pws,
-                              int2* __restrict__ ftable, int* __restrict__ fcend) {
+                              int2* __restrict__ ftable, int* __restrict__ fcend,
+                              int* __restrict__ inv) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int k = (int)perm[s];
     const int id = fid[k];
+    inv[k] = s;
     xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
     vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
     pws[s] = pprev[k];
@@ -815,16 +818,9 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
     }
 }
 
-// Pressure acceleration from the final p, SPH force and the reference integrator
-// (solver.py:24-46):  f = m (a_visc + a^p);  v1 = v0 + (f w + g) dt;  |v1| <= v_max;
-// x1 = x0 + v1 dt.  Writes particle order.
-__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* __restrict__ perm,
-                                                     const int* __restrict__ fid,
-                                                     const float* __restrict__ inv_mass, float dt,
-                                                     float v_max, float* __restrict__ q_out,
-                                                     float* __restrict__ qd_out,
-                                                     float* __restrict__ f_out,
-                                                     float* __restrict__ pprev) {
+// Pressure acceleration from the final p and the SPH force f = m (a_visc + a^p), sorted
+// order: fp[s] = (f, p).
+__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
@@ -833,14 +829,29 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
     press_acc(D, P, w, s, xi, &ax, &ay, &az);
     const float4 av = D.avisc[s];
     const float m = xi.w;
-    const float fx = m * (av.x + ax), fy = m * (av.y + ay), fz = m * (av.z + az);
-    const int k = (int)perm[s];
+    D.fp[s] = make_float4(m * (av.x + ax), m * (av.y + ay), m * (av.z + az), D.p[s]);
+}
+
+// The reference integrator (solver.py:24-46) on fluid particles, in particle order (fluid
+// index k, sorted position inv[k]), so that the particle-order outputs are written
+// coalesced:  v1 = v0 + (f w + g) dt;  |v1| <= v_max;  x1 = x0 + v1 dt.
+__global__ void k_sph_integrate(SphDev D, const int* __restrict__ inv, const int* __restrict__ fid,
+                                const int* __restrict__ fworld, const float* __restrict__ inv_mass,
+                                int n, float dt, float v_max, float* __restrict__ q_out,
+                                float* __restrict__ qd_out, float* __restrict__ f_out,
+                                float* __restrict__ pprev) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int s = inv[k];
     const int id = fid[k];
-    const float wi = inv_mass[id];
+    const PhysWorld P = D.phys[fworld[k]];
+    const float4 f = D.fp[s];
+    const float4 xi = D.xs[s];
     const float4 v0 = D.vs[s];
-    float v1x = v0.x + (fx * wi + P.g[0]) * dt;
-    float v1y = v0.y + (fy * wi + P.g[1]) * dt;
-    float v1z = v0.z + (fz * wi + P.g[2]) * dt;
+    const float wi = inv_mass[id];
+    float v1x = v0.x + (f.x * wi + P.g[0]) * dt;
+    float v1y = v0.y + (f.y * wi + P.g[1]) * dt;
+    float v1z = v0.z + (f.z * wi + P.g[2]) * dt;
     const float mag = sqrtf(dot3(v1x, v1y, v1z, v1x, v1y, v1z));
     if (mag > v_max) {
         const float sc = v_max / mag;
@@ -856,11 +867,11 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const uint32_t* _
     qd_out[3 * id + 1] = v1y;
     qd_out[3 * id + 2] = v1z;
     if (f_out) {
-        f_out[3 * id] = fx;
-        f_out[3 * id + 1] = fy;
-        f_out[3 * id + 2] = fz;
+        f_out[3 * id] = f.x;
+        f_out[3 * id + 1] = f.y;
+        f_out[3 * id + 2] = f.z;
     }
-    pprev[k] = D.p[s];
+    pprev[k] = f.w;
     if (!(isfinite(x1) && isfinite(y1) && isfinite(z1) && isfinite(v1x) && isfinite(v1y) &&
           isfinite(v1z)))
         atomicOr(&D.flags[2], 1);
@@ -1134,11 +1145,12 @@ enum KernelId {
     KT_REDUCE,
     KT_FINAL,
     KT_PASSTHROUGH,
+    KT_INTEGRATE,
     KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source", "mlp_warm_start",
                                             "pressure_acc", "pressure_update", "reduce", "final",
-                                            "passthrough"};
+                                            "passthrough", "integrate"};
 
 struct StepGraph {
     const float* q_in;
@@ -1184,7 +1196,7 @@ struct nsph_solver {
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
-    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, ap, pws, pprev, nbr, ncnt, aux;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags;
     int* h_pinned = nullptr;  // [0] act value
     int stepped = 0;
@@ -1224,6 +1236,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.src = bp<float>(S->src);
     D.p = bp<float>(S->p);
     D.xpr = bp<float4>(S->xpr);
+    D.fp = bp<float4>(S->fp);
     D.mixed = bp<int>(S->mixed);
     D.ap = bp<float4>(S->ap);
     D.pws = bp<float>(S->pws);
@@ -1251,7 +1264,7 @@ static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
                       &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
-                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->ap, &S->pws, &S->pprev,
+                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags};
     for (DevBuf* b : bufs) buf_free(*b);
@@ -1494,6 +1507,8 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->src, nf1 * 4));
     NSPH_TRY(buf_ensure(S->p, nf1 * 4));
     NSPH_TRY(buf_ensure(S->xpr, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->fp, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->inv, nf1 * 4));
     NSPH_TRY(buf_ensure(S->mixed, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
     NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
@@ -1602,7 +1617,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
                         st, &S->skeys, &S->perm));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
                        S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
-                       Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend);
+                       Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));
@@ -1657,10 +1672,13 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, c
                             float* q_out, float* qd_out, float* f_out, float dt) {
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
-    if (S->Nf > 0)
-        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, S->perm,
-                                           bp<int>(S->fid), S->d_inv_mass, dt, S->v_max, q_out,
-                                           qd_out, f_out, bp<float>(S->pprev)));
+    if (S->Nf > 0) {
+        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
+        TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
+                                               bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
+                                               S->d_inv_mass, S->Nf, dt, S->v_max, q_out, qd_out, f_out,
+                                               bp<float>(S->pprev)));
+    }
     if (S->No > 0) {
         TIMED(KT_PASSTHROUGH,
               hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
