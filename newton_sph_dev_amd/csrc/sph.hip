@@ -507,6 +507,7 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 //            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
 // vs[s].w so that the viscosity pass fetches velocity and density together.
+#define NBR_LDS_QUADS 6
 __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -521,14 +522,22 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     int4* __restrict__ out = D.nbr4 + s;
     int c = 0;
     int4 quad = make_int4(0, 0, 0, 0);
-    // whole quads are stored (a partial 4-byte store per slot would write each 64-byte line
-    // back several times); entries shift into the quad from .w, so it is complete, in slot
-    // order, when every fourth entry arrives
+    // Entries shift into the quad from .w, so it is complete, in slot order, when every
+    // fourth entry arrives.  The first NBR_LDS_QUADS quads of a particle wait in LDS
+    // (lq[q * blockDim + t]) and are stored after the search, when the whole wave writes
+    // quad q of its 64 particles at once; storing each quad when it completes would leave
+    // lines partly written and evicted several times.
+    extern __shared__ int4 lq[];
+    const int t = (int)threadIdx.x, bd = (int)blockDim.x;
+    auto put = [&](int q, int4 v) {
+        if (q < NBR_LDS_QUADS) lq[q * bd + t] = v;
+        else out[(size_t)q * Nf] = v;
+    };
     auto append = [&](int j) {
         if (j == s) return;
         if (c < cap) {
             quad = make_int4(quad.y, quad.z, quad.w, j);
-            if ((c & 3) == 3) out[(size_t)(c >> 2) * Nf] = quad;
+            if ((c & 3) == 3) put(c >> 2, quad);
         }
         ++c;
     };
@@ -536,9 +545,11 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     {
         const int cl = c < cap ? c : cap;
         const int u = cl & 3;  // entries of the last, partial quad, at its top
-        if (u == 1) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.w, 0, 0, 0);
-        if (u == 2) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.z, quad.w, 0, 0);
-        if (u == 3) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.y, quad.z, quad.w, 0);
+        if (u == 1) put(cl >> 2, make_int4(quad.w, 0, 0, 0));
+        if (u == 2) put(cl >> 2, make_int4(quad.z, quad.w, 0, 0));
+        if (u == 3) put(cl >> 2, make_int4(quad.y, quad.z, quad.w, 0));
+        const int nq = min((cl + 3) >> 2, NBR_LDS_QUADS);
+        for (int q = 0; q < nq; ++q) out[(size_t)q * Nf] = lq[q * bd + t];
     }
     D.ncnt[s] = c < cap ? c : cap;
     if (c > cap) atomicOr(&D.flags[0], 1);
@@ -1620,7 +1631,8 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
                        Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
-    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, 0, st, D, dt2));
+    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st,
+                                           D, dt2));
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
