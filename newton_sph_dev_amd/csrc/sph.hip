@@ -31,6 +31,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include <vector>
@@ -829,9 +830,62 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
     }
 }
 
+// Step summary over the worlds with fluid (one workgroup): max and sum of the iteration
+// counts, worlds not converged, max density error, the neighbour and finiteness flags, and
+// (act_slot >= 0) the still-iterating count act[act_slot].
+struct RepDev {
+    int act, max_it, sum_it, unconv, overflow, maxnbr, nonfinite, pad;
+    double max_err;
+};
+__global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slot, RepDev* __restrict__ out) {
+    __shared__ int smax[256], ssum[256], sun[256];
+    __shared__ double serr[256];
+    const int t = threadIdx.x;
+    int mx = 0, sm = 0, un = 0;
+    double er = 0.0;
+    for (int w = t; w < W; w += 256) {
+        if (D.w_nf[w] == 0) continue;
+        const int it = D.iters[w];
+        mx = max(mx, it);
+        sm += it;
+        un += D.conv[w] != 1 ? 1 : 0;
+        er = fmax(er, D.err[w]);
+    }
+    smax[t] = mx;
+    ssum[t] = sm;
+    sun[t] = un;
+    serr[t] = er;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+            smax[t] = max(smax[t], smax[t + off]);
+            ssum[t] += ssum[t + off];
+            sun[t] += sun[t + off];
+            serr[t] = fmax(serr[t], serr[t + off]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        RepDev r;
+        r.act = act_slot >= 0 ? D.act[act_slot] : 0;
+        r.max_it = smax[0];
+        r.sum_it = ssum[0];
+        r.unconv = sun[0];
+        r.overflow = D.flags[0];
+        r.maxnbr = D.flags[1];
+        r.nonfinite = D.flags[2];
+        r.pad = 0;
+        r.max_err = serr[0];
+        *out = r;
+    }
+}
+
 // Pressure acceleration from the final p and the SPH force f = m (a_visc + a^p), sorted
-// order: fp[s] = (f, p).
-__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D) {
+// order: fp[s] = (f, p).  With a guard (the still-iterating count of the last iteration),
+// the step's end is enqueued before the host knows whether every world has converged and
+// does nothing unless it has (the host then iterates on and enqueues it again).
+__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const int* __restrict__ guard) {
+    if (guard && *guard != 0) return;
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const PhysWorld P = D.phys[w];
@@ -846,11 +900,12 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D) {
 // The reference integrator (solver.py:24-46) on fluid particles, in particle order (fluid
 // index k, sorted position inv[k]), so that the particle-order outputs are written
 // coalesced:  v1 = v0 + (f w + g) dt;  |v1| <= v_max;  x1 = x0 + v1 dt.
-__global__ void k_sph_integrate(SphDev D, const int* __restrict__ inv, const int* __restrict__ fid,
-                                const int* __restrict__ fworld, const float* __restrict__ inv_mass,
-                                int n, float dt, float v_max, float* __restrict__ q_out,
-                                float* __restrict__ qd_out, float* __restrict__ f_out,
-                                float* __restrict__ pprev) {
+__global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const int* __restrict__ inv,
+                                const int* __restrict__ fid, const int* __restrict__ fworld,
+                                const float* __restrict__ inv_mass, int n, float dt, float v_max,
+                                float* __restrict__ q_out, float* __restrict__ qd_out,
+                                float* __restrict__ f_out, float* __restrict__ pprev) {
+    if (guard && *guard != 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int s = inv[k];
@@ -1202,7 +1257,7 @@ struct nsph_solver {
     // device tables
     DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
     DevBuf fid, fworld, oid;
-    DevBuf btable, bxs, ftable, breach, bcend, fcend;
+    DevBuf btable, bxs, ftable, breach, bcend, fcend, rep;
     DevBuf keys, vals;
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
@@ -1273,7 +1328,7 @@ static int upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
-                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
+                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend, &S->rep,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
@@ -1499,6 +1554,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     }
 
     NSPH_TRY(buf_ensure(S->breach, off + 1));
+    NSPH_TRY(buf_ensure(S->rep, sizeof(RepDev)));
     hipLaunchKernelGGL(k_bnd_reach, dim3((unsigned)((off + 255) / 256)), dim3(256), 0, st, bp<GridWorld>(S->gw), W,
                        (int)off, bp<int2>(S->btable), bp<unsigned char>(S->breach));
     NSPH_HIP(hipGetLastError());
@@ -1536,37 +1592,43 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((nf1 + 255) / 256)), dim3(256), 0, st, bp<float>(S->pprev), nf1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
     NSPH_HIP(hipGetLastError());
-    NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 64, hipHostMallocDefault));
+    NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 128, hipHostMallocDefault));  // [0] act value, [16..] RepDev
     NSPH_HIP(hipStreamSynchronize(st));
     return NSPH_OK;
 }
 
-static int collect_report(nsph_solver* S, nsph_step_report* rep) {
-    hipStream_t st = S->ctx->stream;
-    const int W = S->W;
-    std::vector<int> iters(W), conv(W);
-    std::vector<double> err(W);
-    int fl[4] = {0, 0, 0, 0};
-    NSPH_HIP(hipMemcpyAsync(iters.data(), S->iters.ptr, (size_t)W * 4, hipMemcpyDeviceToHost, st));
-    NSPH_HIP(hipMemcpyAsync(conv.data(), S->conv.ptr, (size_t)W * 4, hipMemcpyDeviceToHost, st));
-    NSPH_HIP(hipMemcpyAsync(err.data(), S->err.ptr, (size_t)W * 8, hipMemcpyDeviceToHost, st));
-    NSPH_HIP(hipMemcpyAsync(fl, S->flags.ptr, 12, hipMemcpyDeviceToHost, st));
+// Enqueue the step summary (k_sph_report into S->rep, one copy into pinned host memory)
+// and wait for it: one stream synchronisation per step.  act_slot selects the
+// still-iterating count that is returned with it (-1: none).
+static int summarise_step(nsph_solver* S, hipStream_t st, int act_slot, RepDev* out) {
+    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(256), 0, st, make_dev(S), S->W, act_slot, bp<RepDev>(S->rep));
+    NSPH_HIP(hipGetLastError());
+    RepDev* h = reinterpret_cast<RepDev*>(S->h_pinned + 16);
+    NSPH_HIP(hipMemcpyAsync(h, S->rep.ptr, sizeof(RepDev), hipMemcpyDeviceToHost, st));
     NSPH_HIP(hipStreamSynchronize(st));
+    *out = *h;
+    return NSPH_OK;
+}
+
+static void report_from(const RepDev& d, nsph_step_report* rep) {
     nsph_step_report r{};
-    for (int w = 0; w < W; ++w) {
-        if (S->h_nf[w] == 0) continue;
-        r.max_iterations = iters[w] > r.max_iterations ? iters[w] : r.max_iterations;
-        r.sum_iterations += iters[w];
-        if (conv[w] != 1) r.unconverged_worlds++;
-        if (err[w] > r.max_density_error) r.max_density_error = err[w];
-    }
-    r.neighbor_overflow = fl[0];
-    r.max_neighbor_count = fl[1];
+    r.max_iterations = d.max_it;
+    r.sum_iterations = d.sum_it;
+    r.unconverged_worlds = d.unconv;
+    r.max_density_error = d.max_err;
+    r.neighbor_overflow = d.overflow;
+    r.max_neighbor_count = d.maxnbr;
     r.status = NSPH_OK;
     if (r.unconverged_worlds) r.status = NSPH_ERR_NOT_CONVERGED;
-    if (fl[0]) r.status = NSPH_ERR_CAPACITY;
-    if (fl[2]) r.status = NSPH_ERR_NONFINITE;
+    if (d.overflow) r.status = NSPH_ERR_CAPACITY;
+    if (d.nonfinite) r.status = NSPH_ERR_NONFINITE;
     *rep = r;
+}
+
+static int collect_report(nsph_solver* S, nsph_step_report* rep) {
+    RepDev d;
+    NSPH_TRY(summarise_step(S, S->ctx->stream, -1, &d));
+    report_from(d, rep);
     return NSPH_OK;
 }
 
@@ -1680,14 +1742,16 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
     return NSPH_OK;
 }
 
+// guard: nullptr, or the still-iterating count that must be 0 for the fluid part to run.
 static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
-                            float* q_out, float* qd_out, float* f_out, float dt) {
+                            float* q_out, float* qd_out, float* f_out, float dt,
+                            const int* guard = nullptr) {
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
     if (S->Nf > 0) {
-        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
+        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
         TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
-                                               bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
+                                               guard, bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
                                                S->d_inv_mass, S->Nf, dt, S->v_max, q_out, qd_out, f_out,
                                                bp<float>(S->pprev)));
     }
@@ -1726,11 +1790,36 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
     if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
     NSPH_TRY(enqueue_prologue(S, st, q_in, qd_in, dt));
     int launched = 0;
-    NSPH_TRY(enqueue_iterations(S, st, dt, 0, S->max_iters, check_every, &launched));
+    if (check_every == 0 || S->Nf == 0) {
+        // no host checks: every iteration slot is enqueued (converged worlds skip them)
+        NSPH_TRY(enqueue_iterations(S, st, dt, 0, S->max_iters, 0, &launched));
+        NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt));
+        S->last_iters_launched = launched;
+        S->stepped = 1;
+        if (report) NSPH_TRY(collect_report(S, report));
+        return NSPH_OK;
+    }
+    // min_iters iterations, then the step's end guarded by "every world converged", so the
+    // common step needs one synchronisation (the summary).  Otherwise iterate on, checking
+    // every check_every iterations, and enqueue the end again.  Iteration counts are decided
+    // on the device per world, so the results do not depend on this schedule.
+    const int first = std::min(std::max(S->min_iters, 1), S->max_iters);
+    NSPH_TRY(enqueue_iterations(S, st, dt, 0, first, 0, &launched));
+    for (;;) {
+        const bool last = launched >= S->max_iters;
+        const int slot = (launched - 1) & 1;
+        NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt,
+                                  last ? nullptr : bp<int>(S->act) + slot));
+        RepDev d;
+        NSPH_TRY(summarise_step(S, st, last ? -1 : slot, &d));
+        if (last || d.act == 0) {
+            if (report) report_from(d, report);
+            break;
+        }
+        NSPH_TRY(enqueue_iterations(S, st, dt, launched, S->max_iters, check_every, &launched));
+    }
     S->last_iters_launched = launched;
-    NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt));
     S->stepped = 1;
-    if (report) NSPH_TRY(collect_report(S, report));
     return NSPH_OK;
 }
 
