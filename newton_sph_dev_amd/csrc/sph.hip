@@ -232,24 +232,6 @@ __device__ inline void mark_cell(const uint32_t* __restrict__ skeys, int s, int 
     }
 }
 
-// Clear the cells that the previous build occupied (its sorted keys name them).
-__global__ void k_sph_clear_prev(const uint32_t* __restrict__ prev_keys, int n, int2* __restrict__ table) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    table[prev_keys[s] >> NSPH_XSUB_BITS] = make_int2(0, 0);
-}
-
-__global__ void k_sph_keys(const float* __restrict__ q, const int* __restrict__ fid,
-                           const int* __restrict__ fworld, const GridWorld* __restrict__ gw,
-                           int n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int id = fid[k];
-    const GridWorld g = gw[fworld[k]];
-    keys[k] = slab_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
-    vals[k] = (uint32_t)k;
-}
-
 __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restrict__ qd,
                               const float* __restrict__ mass, const int* __restrict__ fid,
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
@@ -268,20 +250,35 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
     mark_cell(skeys, s, n, ftable, fcend);
 }
 
-__global__ void k_sph_step_init(SphDev D, int W) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < W) {
-        D.iters[w] = 0;
-        D.err[w] = 0.0;
-        D.mixed[w] = 0;
-        D.conv[w] = D.w_nf[w] == 0 ? 1 : 0;
+// Start of a step, one thread per world and per fluid particle (fluid index k):
+//   worlds:    iteration counters and flags cleared
+//   particles: the cell of k's previous sorted key (prev_keys, zero-filled before the first
+//              build) is cleared, and k's sort key and value are written.
+// prev_keys may be the keys array itself; thread k reads entry k before writing it.
+__global__ void k_sph_begin(SphDev D, int W, int n, const uint32_t* prev_keys, int2* __restrict__ table,
+                            const float* __restrict__ q, const int* __restrict__ fid,
+                            const int* __restrict__ fworld, const GridWorld* __restrict__ gw,
+                            uint32_t* keys, uint32_t* __restrict__ vals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < W) {
+        D.iters[k] = 0;
+        D.err[k] = 0.0;
+        D.mixed[k] = 0;
+        D.conv[k] = D.w_nf[k] == 0 ? 1 : 0;
     }
-    if (w == 0) {
+    if (k == 0) {
         D.act[0] = 0;
         D.act[1] = W;
         D.flags[0] = 0;
         D.flags[1] = 0;
         D.flags[2] = 0;
+    }
+    if (k < n) {
+        table[prev_keys[k] >> NSPH_XSUB_BITS] = make_int2(0, 0);
+        const int id = fid[k];
+        const GridWorld g = gw[fworld[k]];
+        keys[k] = slab_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+        vals[k] = (uint32_t)k;
     }
 }
 
@@ -1676,16 +1673,12 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     const int Nf = S->Nf, W = S->W, C = S->C;
     const float dt2 = (float)((double)dt * (double)dt);
     SphDev D = make_dev(S);
-    hipLaunchKernelGGL(k_sph_step_init, dim3((W + 255) / 256), dim3(256), 0, st, D, W);
-    if (Nf == 0) return NSPH_OK;
-    EventPair* ep_build;
-    NSPH_TRY(tic(S, KT_BUILD, &ep_build));
-    // the sorted keys of the previous build (zero-filled at creation) name the cells to clear
-    hipLaunchKernelGGL(k_sph_clear_prev, dim3((Nf + 255) / 256), dim3(256), 0, st,
-                       S->skeys, Nf, bp<int2>(S->ftable));
-    hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in,
-                       bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw), Nf,
+    EventPair* ep_build = nullptr;
+    if (Nf > 0) NSPH_TRY(tic(S, KT_BUILD, &ep_build));
+    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(Nf, W) + 255) / 256), dim3(256), 0, st, D, W, Nf, S->skeys,
+                       bp<int2>(S->ftable), q_in, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
                        bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+    if (Nf == 0) return NSPH_OK;
     NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
                         st, &S->skeys, &S->perm));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
