@@ -200,9 +200,13 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* model, const nsph_s
 /* One step: state_in (d_q_in, d_qd_in) -> state_out (d_q_out, d_qd_out, d_f_out).
  * d_f_out receives the SPH force (viscosity + pressure, N) of fluid particles and 0
  * elsewhere; it may be NULL. Inactive particles: q_out = q_in, qd_out untouched
- * (solver.py:27-29). check_every: read the convergence flag on the host every
- * check_every Jacobi iterations (0 = launch max_iters early-exit iterations without any
- * host read; the step is then capture-safe). report may be NULL (no host sync). */
+ * (solver.py:27-29). check_every > 0: after min_iters Jacobi iterations the step's end is
+ * enqueued guarded by "every world converged" and the host synchronises once; only if a
+ * world has not converged does it iterate on, reading the convergence count every
+ * check_every iterations, and enqueue the end again (iteration counts are decided per world
+ * on the device, so results do not depend on this schedule). check_every = 0: launch
+ * max_iters early-exit iterations without any host read; the step is then capture-safe.
+ * report may be NULL; with check_every = 0 and no report the step does not synchronise. */
 int nsph_solver_step(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
                      float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
                      int check_every, nsph_step_report* report);
