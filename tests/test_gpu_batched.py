@@ -106,6 +106,32 @@ def test_launch_configs_do_not_change_results():
         assert np.array_equal(q, outs[0][0]) and np.array_equal(qd, outs[0][1])
 
 
+def test_check_schedules_do_not_change_results():
+    """The step's end is enqueued after min_iters iterations behind a device-side "every world
+    converged" guard; at dt = 1e-3 with the zero warm start every step needs more iterations,
+    so the guarded end is skipped and enqueued again.  Host check intervals 1 and 3 and no
+    host check at all (0) must give identical steps and iteration counts."""
+    cfg = scenes.small_batch(6, seed=9)
+    _print("check_every 0/1/3 at dt = 1e-3: identical steps")
+    b, H = scenes.build_batched(cfg)
+    m = b.finalize("cuda:0")
+    outs = []
+    for ce in (0, 1, 3):
+        s = SolverSPH(m, H, warm_start="zero", check_every=ce, min_iters=2)
+        a, c = m.state(), m.state()
+        iters = []
+        for _ in range(3):
+            s.step(a, c, None, None, 1e-3)
+            iters.append(s.last_report["max_iterations"])
+            a, c = c, a
+        torch.cuda.synchronize()
+        outs.append((a.particle_q.cpu().numpy(), a.particle_qd.cpu().numpy(), iters))
+    assert min(outs[0][2]) > 2
+    for q, qd, it in outs[1:]:
+        assert it == outs[0][2]
+        assert np.array_equal(q, outs[0][0]) and np.array_equal(qd, outs[0][1])
+
+
 def test_autotune_picks_a_config(tmp_path):
     from newton_sph_dev_amd import autotune
 
