@@ -10,6 +10,8 @@ checked against something independent of its own formula:
     A p = first-order change of the SPH density sum when particles move by dt^2 a^p(p);
   - pressure / viscosity forces: momentum conservation (sum of m a = 0) without boundary;
   - integrator: closed-form free fall and velocity clamp (solver.py:24-46);
+  - hydrostatic column: after settling, gravity balanced by the SPH force, density rho0 and
+    pressure rising at rho0 g with depth in the interior;
   - FP32 restatement vs FP64 oracle: within the stated tolerances.
 """
 
@@ -279,3 +281,15 @@ def test_world_reduce_is_deterministic_and_accurate():
     rs = O.step(pbs, pbs.q0, pbs.qd0, precision="f32")
     assert np.array_equal(r["q"][:n], rs["q"])
     assert r["iters"][0] == rs["iters"][0] and r["err"][0] == rs["err"][0]
+
+
+def test_hydrostatic_column_settles_on_the_oracle():
+    import hydrostatic as HC
+
+    m = HC.column("cpu")
+    pb = O.problem_from_model(m, SPHParams.resolve(m, 2 * HC.S, **HC.PARAMS))
+    q, qd, pp = pb.q0, pb.qd0, None
+    for _ in range(HC.STEPS):
+        r = O.step(pb, q, qd, pprev=pp, dt=HC.DT, precision="f32")
+        q, qd, pp = r["q"], r["qd"], r["pprev"]
+    HC.check(HC.metrics(q, qd, r["p"], r["rho"], r["f"], pb.mass, pb.inv_mass > 0))
