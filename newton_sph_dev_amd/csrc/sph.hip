@@ -9,7 +9,7 @@
 // x1 = x0 + v1*dt, inactive particles keep x and leave v untouched.
 //
 // Layout (HBM, fluid particles in cell-sorted order s; one world = one contiguous run):
-//   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, 0)
+//   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, rho)
 //   rho[s], aii[s], src[s], p[s]   float
 //   xpr[s]  float4 (x, y, z, p/rho^2)   (positions again, so that the pressure-acceleration
 //                                     gather of a world of one fluid mass loads 16 bytes)
@@ -17,9 +17,14 @@
 //   avisc[s], ap[s]  float4      vadv[s] float4 (v_adv, pr0), pr0 = 0.5 p_prev / rho^2 under the
 //                                previous-pressure warm start, else 0
 //   nbr4[q*Nf + s] int4  fluid-fluid neighbour list, slots 4q..4q+3 of particle s (lanes
-//                        of a wave read 64 consecutive quads)
+//                        of a wave read 64 consecutive quads); ascending in s
+//   fp[s]   float4 (SPH force, p) at the step's end;  inv[k] sorted position of fluid index k
 // Boundary particles are hashed once into their own table (bxs, btable); only the density
 // pass visits them, and it folds them into gb, so the other passes never touch them.
+//
+// Each neighbour gather pass is bound by the L1's cache-line accesses (a lane's load of a
+// neighbour's data is one access), so the layouts above keep the loads per neighbour low:
+// one float4 for density and pressure acceleration, two for the other passes.
 //
 // Work partition: a "chunk" is 256 consecutive sorted fluid particles of one world.
 // Kernels run one workgroup per chunk, so the world of a workgroup is uniform and every
