@@ -192,6 +192,8 @@ struct GridView {
     const int2* table;
     const uint32_t* perm;
     int n;
+    float4* px;  // scratch of n entries for the consumers' packed data in sorted order
+    float4* pv;
 };
 
 }  // namespace nsph

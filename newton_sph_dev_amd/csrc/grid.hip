@@ -144,6 +144,7 @@ struct nsph_grid {
     float radius = 0.f;
     GridWorld g{};
     DevBuf keys, vals, xs, table, flag;
+    DevBuf px, pv;  // consumers' packed per-particle data in sorted order (scratch)
     uint32_t* sorted_keys = nullptr;  // result buffers of the last sort (keys / perm)
     uint32_t* perm = nullptr;
     int n_prev = 0;  // points of the previous build (their cells are cleared next build)
@@ -157,6 +158,8 @@ int nsph_grid_view(const nsph_grid* g, GridView* out) {
     out->table = bp<int2>(g->table);
     out->perm = g->perm;
     out->n = g->n;
+    out->px = bp<float4>(g->px);
+    out->pv = bp<float4>(g->pv);
     return NSPH_OK;
 }
 
@@ -235,6 +238,8 @@ int nsph_grid_reserve(nsph_grid* g, int n) {
     NSPH_TRY(buf_ensure(g->keys, (size_t)n * 4));
     NSPH_TRY(buf_ensure(g->vals, (size_t)n * 4));
     NSPH_TRY(buf_ensure(g->xs, (size_t)n * 16));
+    NSPH_TRY(buf_ensure(g->px, (size_t)n * 16));
+    NSPH_TRY(buf_ensure(g->pv, (size_t)n * 16));
     NSPH_TRY(radix_reserve(g->rs, n));
     g->reserved = n;
     if (g->n_prev > 0) {
@@ -358,6 +363,8 @@ int nsph_grid_destroy(nsph_grid* g) {
     buf_free(g->xs);
     buf_free(g->table);
     buf_free(g->flag);
+    buf_free(g->px);
+    buf_free(g->pv);
     radix_free(g->rs);
     delete g;
     return NSPH_OK;
