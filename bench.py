@@ -1,6 +1,7 @@
 """Benchmark: SPH dam break, 1,048,576 fluid particles, one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dam_break_1m|dam_break_32k]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload dam_break_1m|dam_break_32k|batched_4096|batched_1024|xpbd_granular_1m|semi_implicit_1m]
                     [--warm-start zero|prev] [--no-cpu-baseline]
 
 Prints ONE JSON line. A step = one SolverSPH.step (hash/sort, neighbour lists, density,
@@ -107,6 +108,65 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
     }
 
 
+CONTACT_WORKLOADS = ("xpbd_granular_1m", "semi_implicit_1m")
+
+
+def bench_contacts(args) -> None:
+    """The reference's own particle-contact paths on the HIP grid: a 100^3 block of grains
+    (radius 1 cm) falling onto a floor of static grains, 1,044,100 particles.  xpbd_granular_1m
+    steps SolverXPBD (4 iterations: grid build, particle contacts, deltas); semi_implicit_1m
+    builds model.particle_grid each step (as a user of the reference does) and steps
+    SolverSemiImplicit (penalty contacts, integration).  One step = one solver.step."""
+    import torch
+
+    from newton_sph_dev_amd import ModelBuilder, SolverSemiImplicit, SolverXPBD
+
+    n, r = 100, 0.01
+    b = ModelBuilder()
+    b.add_particle_grid(pos=(0.1, 0.1, 0.05), rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0), dim_x=n, dim_y=n,
+                        dim_z=n, cell_x=2.1 * r, cell_y=2.1 * r, cell_z=2.1 * r, mass=0.01, jitter=0.2 * r,
+                        radius_mean=r)
+    g = np.stack(np.meshgrid(np.arange(2 * n + 10), np.arange(2 * n + 10), indexing="ij"), -1).reshape(-1, 2)
+    floor = np.zeros((g.shape[0], 3))
+    floor[:, :2] = g * (2.0 * r)
+    b.add_particles(floor, np.zeros_like(floor), np.zeros(floor.shape[0]), np.full(floor.shape[0], r))
+    m = b.finalize("cuda:0")
+    m.particle_mu = 0.5
+    xpbd = args.workload == "xpbd_granular_1m"
+    solver = SolverXPBD(m, iterations=4) if xpbd else SolverSemiImplicit(m)
+    s0, s1 = m.state(), m.state()
+    dt = args.dt if args.dt > 0 else 1.0 / 60.0 / 8
+
+    def step():
+        nonlocal s0, s1
+        s0.clear_forces()
+        if not xpbd:
+            m.particle_grid.build(s0.particle_q, radius=m.particle_max_radius * 2.0 + m.particle_cohesion)
+        solver.step(s0, s1, None, None, dt)
+        s0, s1 = s1, s0
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ta = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ta) * 1e3)
+    el = time.perf_counter() - t0
+    npart = m.particle_count
+    print(json.dumps({
+        "metric": "particle-updates/s (" + ("XPBD" if xpbd else "semi-implicit") + " particle step, 1M grains)",
+        "value": npart * args.steps / el, "unit": "particle-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded grain lattice)",
+        "config": {"workload": args.workload, "particles": npart, "dt": dt, "iterations": 4 if xpbd else None},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "roofline": None, "cpu_baseline": None}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +194,10 @@ def main():
     import torch
 
     from newton_sph_dev_amd import SolverSPH, _native as N, scenes
+
+    if args.workload in CONTACT_WORKLOADS:
+        bench_contacts(args)
+        return
 
     if args.workload.startswith("batched"):
         cfg = {"batched_4096": scenes.BATCH_4096, "batched_1024": scenes.BATCH_1024}[args.workload]

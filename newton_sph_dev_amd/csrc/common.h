@@ -194,6 +194,7 @@ struct GridView {
     int n;
     float4* px;  // scratch of n entries for the consumers' packed data in sorted order
     float4* pv;
+    const int* inv;  // sorted position of each point
 };
 
 }  // namespace nsph

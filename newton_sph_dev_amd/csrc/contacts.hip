@@ -47,13 +47,14 @@ __device__ inline void grid_candidates(const GridView& G, float qx, float qy, fl
 
 __device__ inline float3 ld3(const float* p, int i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
-// px[s] = (x, r) of point perm[s] (NaN for an inactive one), pv[s] = (v, inverse mass or 0).
+// px[s] = (x, r) of point i (NaN for an inactive one), pv[s] = (v, inverse mass or 0), s = inv[i]:
+// read in point order (coalesced), written to the sorted positions.
 __global__ void k_pack_particles(GridView G, const float* __restrict__ x, const float* __restrict__ v,
                                  const float* __restrict__ radius, const float* __restrict__ invmass,
                                  const int* __restrict__ flags) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= G.n) return;
-    const int i = (int)G.perm[s];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    const int s = G.inv[i];
     const float nan = __int_as_float(0x7fc00000);
     G.px[s] = (flags[i] & NSPH_PARTICLE_ACTIVE) ? make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], radius[i])
                                                  : make_float4(nan, nan, nan, nan);

@@ -80,12 +80,14 @@ __global__ void k_grid_keys(const float* __restrict__ pts, int n, GridWorld g,
     vals[i] = (uint32_t)i;
 }
 
+// Build positions in sorted order, and the inverse permutation (sorted position of each point).
 __global__ void k_grid_reorder(const float* __restrict__ pts, const uint32_t* __restrict__ perm,
-                               int n, float4* __restrict__ xs) {
+                               int n, float4* __restrict__ xs, int* __restrict__ inv) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t id = perm[s];
     xs[s] = make_float4(pts[3 * id], pts[3 * id + 1], pts[3 * id + 2], 0.0f);
+    inv[id] = s;
 }
 
 __global__ void k_grid_neighbors(GridWorld g, const int2* __restrict__ table,
@@ -145,6 +147,7 @@ struct nsph_grid {
     GridWorld g{};
     DevBuf keys, vals, xs, table, flag;
     DevBuf px, pv;  // consumers' packed per-particle data in sorted order (scratch)
+    DevBuf inv;     // sorted position of each point of the last build
     uint32_t* sorted_keys = nullptr;  // result buffers of the last sort (keys / perm)
     uint32_t* perm = nullptr;
     int n_prev = 0;  // points of the previous build (their cells are cleared next build)
@@ -159,6 +162,7 @@ int nsph_grid_view(const nsph_grid* g, GridView* out) {
     out->perm = g->perm;
     out->n = g->n;
     out->px = bp<float4>(g->px);
+    out->inv = bp<int>(g->inv);
     out->pv = bp<float4>(g->pv);
     return NSPH_OK;
 }
@@ -240,6 +244,7 @@ int nsph_grid_reserve(nsph_grid* g, int n) {
     NSPH_TRY(buf_ensure(g->xs, (size_t)n * 16));
     NSPH_TRY(buf_ensure(g->px, (size_t)n * 16));
     NSPH_TRY(buf_ensure(g->pv, (size_t)n * 16));
+    NSPH_TRY(buf_ensure(g->inv, (size_t)n * 4));
     NSPH_TRY(radix_reserve(g->rs, n));
     g->reserved = n;
     if (g->n_prev > 0) {
@@ -283,7 +288,7 @@ int nsph_grid_build(nsph_grid* g, const float* d_points, int n, float radius) {
         NSPH_TRY(radix_sort(g->rs, bp<uint32_t>(g->keys), bp<uint32_t>(g->vals), n,
                             bits_for(max_key), st, &g->sorted_keys, &g->perm));
         hipLaunchKernelGGL(k_grid_reorder, dim3((n + 255) / 256), dim3(256), 0, st, d_points,
-                           g->perm, n, bp<float4>(g->xs));
+                           g->perm, n, bp<float4>(g->xs), bp<int>(g->inv));
         hipLaunchKernelGGL(k_cells_mark, dim3((n + 255) / 256), dim3(256), 0, st, g->sorted_keys,
                            n, bp<int2>(g->table));
     }
@@ -365,6 +370,7 @@ int nsph_grid_destroy(nsph_grid* g) {
     buf_free(g->flag);
     buf_free(g->px);
     buf_free(g->pv);
+    buf_free(g->inv);
     radix_free(g->rs);
     delete g;
     return NSPH_OK;
