@@ -140,7 +140,9 @@ struct SphDev {
     double* err;
     int* conv;
     int* act;
-    int* flags;  // [0] overflow, [1] max count, [2] non-finite
+    int* flags;  // [0] overflow, [2] non-finite
+    int* wmax;   // per wave of the neighbour kernel: its largest uncapped count
+    int nwave;   // 4 per chunk
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. Renumber them
@@ -275,9 +277,9 @@ __global__ void k_sph_begin(SphDev D, int W, int n, const uint32_t* prev_keys, i
         D.act[0] = 0;
         D.act[1] = W;
         D.flags[0] = 0;
-        D.flags[1] = 0;
         D.flags[2] = 0;
     }
+    if (k < D.nwave) D.wmax[k] = 0;
     if (k < n) {
         table[prev_keys[k] >> NSPH_XSUB_BITS] = make_int2(0, 0);
         const int id = fid[k];
@@ -387,10 +389,11 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
         int2 cn[3];
         int fan = 0, fbn = 0, rown = 0;
         const bool okn = k < 8 && load_row(k + 1, cn, &fan, &fbn, &rown);
-        if (ok) {
-            const int2 rr = row_slab_range(c, nx, x0, fa, fb, cend, row);
-            scan(rr.x, rr.y);
-        }
+        // every lane calls scan (an empty range for a skipped row), so a scan may exchange
+        // data between lanes
+        int2 rr = make_int2(0, 0);
+        if (ok) rr = row_slab_range(c, nx, x0, fa, fb, cend, row);
+        scan(rr.x, rr.y);
         ok = okn;
         fa = fan;
         fb = fbn;
@@ -404,6 +407,7 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
 // calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).  Positions are fetched RV_BATCH
 // at a time.
 #define RV_BATCH 4
+#define XS_PAD 8  // positions readable past the last fluid particle
 template <class F>
 __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table, const int* __restrict__ cend,
                                   const float4* __restrict__ xs, float qx, float qy, float qz,
@@ -425,40 +429,128 @@ __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ t
     });
 }
 
+// Lane exchange within groups of four lanes (DPP quad_perm): lane l reads lane
+// (l & ~3) + sel[l & 3] of v.
+#define QPERM(a, b, c, d) ((a) | ((b) << 2) | ((c) << 4) | ((d) << 6))
+template <int CTRL>
+__device__ inline int quad_perm(int v) {
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL>
+__device__ inline float quad_perm(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// Maximum of v over the 16 lanes of each DPP row, in every lane of the row (all lanes of the
+// wave active): two quad exchanges, then rotations by 4 and 8 lanes within the row.
+__device__ inline int row16_max(int v) {
+    v = max(v, quad_perm<QPERM(1, 0, 3, 2)>(v));
+    v = max(v, quad_perm<QPERM(2, 3, 0, 1)>(v));
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x124, 0xF, 0xF, true));  // row_ror:4
+    v = max(v, __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, true));  // row_ror:8
+    return v;
+}
+
+// Accept bit of candidate (px, py, pz) held by lane U of the quad, shifted into m.
+template <int U>
+__device__ inline void quad_test(uint32_t& m, float px, float py, float pz, float qx, float qy, float qz,
+                                 float r2max) {
+    constexpr int C = QPERM(U, U, U, U);
+    const float dx = qx - quad_perm<C>(px), dy = qy - quad_perm<C>(py), dz = qz - quad_perm<C>(pz);
+    const float d2 = (dx * dx + dy * dy) + dz * dz;
+    m = m + m + (d2 < r2max ? 1u : 0u);
+}
+
 // rows_visit for callers that need only the accepted indices, in the same order: each row
 // range is tested in windows of 32 candidates into an accept bitmask, and f(j) is then
 // called for the accepted candidates in ascending order.  The per-candidate work of a wave
 // no longer includes the caller's (divergent) accept code.  Candidate t of a window of n
-// enters the mask at bit n4 - 1 - t (m = 2m + accept; n4 = n rounded up to RV_BATCH), so
-// each test costs a compare and an add-with-carry; a window reads up to RV_BATCH - 1
-// positions past its row, which xs must hold (padding) and which are masked off.
+// enters the mask at bit n4 - 1 - t (m = 2m + accept; n4 = n rounded up to the batch: 4 per
+// lane, 8 per quad), so each test costs a compare and an add-with-carry; a window reads up to
+// XS_PAD - 1 positions past its range, which xs must hold (padding) and which are masked off.
+//
+// The four lanes of a quad are consecutive sorted particles, nearly always of one cell or
+// of neighbouring cells of one cell row, so their ranges of a row overlap.  When every
+// quad's ranges of the row overlap (their union is at most 12 points longer than the
+// longest), the quad scans the union together: lane u of the quad loads candidate 4k + u
+// and each lane reads the four candidates from the quad's registers, so one load brings
+// four candidates instead of one.  Each lane then keeps the bits of its own range, so the
+// accepted indices and their order are those of the per-lane scan.  The quad loads two
+// groups of four per step, so two loads are in flight.  A wave with inactive
+// lanes, or a row where some quad's ranges are disjoint (a quad that straddles two cell
+// rows), scans per lane.
 template <class F>
 __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
                                       const int* __restrict__ cend,
                                       const float4* __restrict__ xs, float qx, float qy, float qz,
                                       float r, float r2max, F f) {
+    const bool full = __builtin_amdgcn_read_exec() == ~0ull;
+    const int lq = (int)(__lane_id() & 3u);
+    auto extract = [&](uint32_t m, int base, int n4) {
+        while (m != 0u) {
+            const int b = 31 - __builtin_clz(m);
+            m ^= 1u << b;
+            f(base + (n4 - 1 - b));
+        }
+    };
     rows_for_each(g, table, cend, qx, qy, qz, r, [&](int lo, int hi) {
-        for (int base = lo; base < hi; base += 32) {
-            const int n = min(hi - base, 32);
-            const float4* __restrict__ xb = xs + base;
-            uint32_t m = 0u;
-            int n4 = 0;
-            for (; n4 < n; n4 += RV_BATCH) {
-                float4 p[RV_BATCH];
-#pragma unroll
-                for (int v = 0; v < RV_BATCH; ++v) p[v] = xb[n4 + v];
-#pragma unroll
-                for (int v = 0; v < RV_BATCH; ++v) {
-                    const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
-                    const float d2 = (dx * dx + dy * dy) + dz * dz;
-                    m = m + m + (d2 < r2max ? 1u : 0u);
+        bool coop = false;
+        int ulo = 0, uhi = 0;
+        if (full) {
+            const bool has = lo < hi;
+            int a = has ? lo : 0x7fffffff, b = has ? hi : -1, len = has ? hi - lo : 0;
+            a = min(a, quad_perm<QPERM(1, 0, 3, 2)>(a));
+            a = min(a, quad_perm<QPERM(2, 3, 0, 1)>(a));
+            b = max(b, quad_perm<QPERM(1, 0, 3, 2)>(b));
+            b = max(b, quad_perm<QPERM(2, 3, 0, 1)>(b));
+            len = max(len, quad_perm<QPERM(1, 0, 3, 2)>(len));
+            len = max(len, quad_perm<QPERM(2, 3, 0, 1)>(len));
+            const bool tight = len == 0 || b - a <= len + 12;
+            coop = __ballot(!tight) == 0ull;
+            ulo = len == 0 ? 0 : a;
+            uhi = len == 0 ? 0 : b;
+        }
+        if (coop) {
+            // quad-uniform trip counts: the quad's lanes stay active together
+            for (int base = ulo; base < uhi; base += 32) {
+                const int n = min(uhi - base, 32);
+                uint32_t m = 0u;
+                int n4 = 0;
+                for (; n4 < n; n4 += 8) {
+                    const float4 p = xs[base + n4 + lq], p2 = xs[base + n4 + 4 + lq];
+                    quad_test<0>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
+                    quad_test<1>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
+                    quad_test<2>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
+                    quad_test<3>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
+                    quad_test<0>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
+                    quad_test<1>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
+                    quad_test<2>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
+                    quad_test<3>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
                 }
+                // this lane's own range within the window: positions [ka, kb)
+                const int ka = max(lo - base, 0), kb = min(hi - base, n);
+                m &= kb > ka ? (uint32_t)(((1ull << (kb - ka)) - 1ull) << (n4 - kb)) : 0u;
+                extract(m, base, n4);
             }
-            m &= ~((1u << (n4 - n)) - 1u);  // the tail past the row
-            while (m != 0u) {
-                const int b = 31 - __builtin_clz(m);
-                m ^= 1u << b;
-                f(base + (n4 - 1 - b));
+        } else {
+            for (int base = lo; base < hi; base += 32) {
+                const int n = min(hi - base, 32);
+                const float4* __restrict__ xb = xs + base;
+                uint32_t m = 0u;
+                int n4 = 0;
+                for (; n4 < n; n4 += RV_BATCH) {
+                    float4 p[RV_BATCH];
+#pragma unroll
+                    for (int v = 0; v < RV_BATCH; ++v) p[v] = xb[n4 + v];
+#pragma unroll
+                    for (int v = 0; v < RV_BATCH; ++v) {
+                        const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
+                        const float d2 = (dx * dx + dy * dy) + dz * dz;
+                        m = m + m + (d2 < r2max ? 1u : 0u);
+                    }
+                }
+                m &= ~((1u << (n4 - n)) - 1u);  // the tail past the row
+                extract(m, base, n4);
             }
         }
     });
@@ -556,7 +648,21 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
     }
     D.ncnt[s] = c < cap ? c : cap;
     if (c > cap) atomicOr(&D.flags[0], 1);
-    atomicMax(&D.flags[1], c);
+    // Largest count of the step: each wave stores its own and k_sph_report takes the maximum.
+    // (A per-lane atomicMax on one address compiles to a scalar loop over the wave's lanes,
+    // about a tenth of this kernel's time; one same-address atomic per wave still costs
+    // several percent across the eight XCDs.)
+    {
+        const int wv = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+        if (__ballot(1) == ~0ull) {
+            const int r = row16_max(c);
+            const int mx = max(max(__builtin_amdgcn_readlane(r, 0), __builtin_amdgcn_readlane(r, 16)),
+                               max(__builtin_amdgcn_readlane(r, 32), __builtin_amdgcn_readlane(r, 48)));
+            if (__lane_id() == 0u) D.wmax[wv] = mx;
+        } else {
+            atomicMax(&D.wmax[wv], c);  // a wave with inactive lanes (the end of a world)
+        }
+    }
 
     float sum = xi.w * P.W0, cmp = 0.0f;
     float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
@@ -840,11 +946,12 @@ struct RepDev {
     double max_err;
 };
 __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slot, RepDev* __restrict__ out) {
-    __shared__ int smax[256], ssum[256], sun[256];
+    __shared__ int smax[256], ssum[256], sun[256], snb[256];
     __shared__ double serr[256];
     const int t = threadIdx.x;
-    int mx = 0, sm = 0, un = 0;
+    int mx = 0, sm = 0, un = 0, nb = 0;
     double er = 0.0;
+    for (int i = t; i < D.nwave; i += 256) nb = max(nb, D.wmax[i]);
     for (int w = t; w < W; w += 256) {
         if (D.w_nf[w] == 0) continue;
         const int it = D.iters[w];
@@ -856,6 +963,7 @@ __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slo
     smax[t] = mx;
     ssum[t] = sm;
     sun[t] = un;
+    snb[t] = nb;
     serr[t] = er;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -863,6 +971,7 @@ __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slo
             smax[t] = max(smax[t], smax[t + off]);
             ssum[t] += ssum[t + off];
             sun[t] += sun[t + off];
+            snb[t] = max(snb[t], snb[t + off]);
             serr[t] = fmax(serr[t], serr[t + off]);
         }
         __syncthreads();
@@ -874,7 +983,7 @@ __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slo
         r.sum_it = ssum[0];
         r.unconv = sun[0];
         r.overflow = D.flags[0];
-        r.maxnbr = D.flags[1];
+        r.maxnbr = snb[0];
         r.nonfinite = D.flags[2];
         r.pad = 0;
         r.max_err = serr[0];
@@ -1265,7 +1374,7 @@ struct nsph_solver {
     uint32_t* perm = nullptr;
     RadixSort rs;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, pprev, nbr, ncnt, aux;
-    DevBuf partial, iters, err, conv, act, flags;
+    DevBuf partial, iters, err, conv, act, flags, wmax;
     int* h_pinned = nullptr;  // [0] act value
     int stepped = 0;
     int last_iters_launched = 0;
@@ -1317,6 +1426,8 @@ static SphDev make_dev(nsph_solver* S) {
     D.conv = bp<int>(S->conv);
     D.act = bp<int>(S->act);
     D.flags = bp<int>(S->flags);
+    D.wmax = bp<int>(S->wmax);
+    D.nwave = 4 * S->C;
     return D;
 }
 
@@ -1334,7 +1445,7 @@ static void solver_free(nsph_solver* S) {
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
-                      &S->flags};
+                      &S->flags, &S->wmax};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
     if (S->h_pinned) (void)hipHostFree(S->h_pinned);
@@ -1566,7 +1677,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->keys), (size_t)(nmax + 1), 0);
     S->skeys = bp<uint32_t>(S->keys);
     const size_t nf1 = (size_t)Nf + 1;
-    NSPH_TRY(buf_ensure(S->xs, (nf1 + RV_BATCH) * 16));  // windows read past a row's end
+    NSPH_TRY(buf_ensure(S->xs, (nf1 + XS_PAD) * 16));  // windows read past a row's end
     NSPH_TRY(buf_ensure(S->vs, nf1 * 16));
     NSPH_TRY(buf_ensure(S->rho, nf1 * 4));
     NSPH_TRY(buf_ensure(S->aii, nf1 * 4));
@@ -1586,6 +1697,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->ncnt, nf1 * 4));
     NSPH_TRY(buf_ensure(S->aux, nf1 * 8));
     NSPH_TRY(buf_ensure(S->partial, ((size_t)S->C + 1) * 8));
+    NSPH_TRY(buf_ensure(S->wmax, ((size_t)S->C + 1) * 16));
     NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->err, ((size_t)W + 1) * 8));
     NSPH_TRY(buf_ensure(S->conv, ((size_t)W + 1) * 4));
@@ -1680,7 +1792,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     SphDev D = make_dev(S);
     EventPair* ep_build = nullptr;
     if (Nf > 0) NSPH_TRY(tic(S, KT_BUILD, &ep_build));
-    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(Nf, W) + 255) / 256), dim3(256), 0, st, D, W, Nf, S->skeys,
+    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(std::max(Nf, W), D.nwave) + 255) / 256), dim3(256), 0, st, D, W, Nf, S->skeys,
                        bp<int2>(S->ftable), q_in, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
                        bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
     if (Nf == 0) return NSPH_OK;

@@ -116,6 +116,8 @@ def test_step_small_dam_break_parity(dims):
     assert np.array_equal(oc, counts)
     for i in fid:
         assert np.array_equal(oi[: oc[i], i], idx[: counts[i], i])
+    # the step report's largest list (1,680 fluid particles end in a partial wave, 7,680 do not)
+    assert rep["max_neighbor_count"] == int(oc[fid].max())
 
     r32 = O.step(pb, q0, qd0, precision="f32")
     assert int(r32["iters"][0]) == rep["max_iterations"]
