@@ -559,15 +559,13 @@ __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict
 // Gather helper: for each list entry of s in slot order, load(j) fetches the neighbour data
 // (B = 4 or 8 entries, one or two quads, at a time) and use(data, j) accumulates in slot
 // order.  Missing entries of the last batch repeat the first one and are not used.
-template <class T, int B, class Load, class Use>
-__device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) {
+// quad(q) returns list quad q of s.
+template <class T, int B, class Quad, class Load, class Use>
+__device__ inline void for_each_nbr_q(int n, Quad quad, Load load, Use use) {
     static_assert(B == 4 || B == 8, "batch of 4 or 8");
-    const int n = D.ncnt[s];
-    const int4* __restrict__ L = D.nbr4 + s;
-    const size_t stride = (size_t)D.Nf;
     for (int k = 0; k < n; k += B) {
         const int m = n - k;
-        const int4 q0 = L[(size_t)(k >> 2) * stride];
+        const int4 q0 = quad(k >> 2);
         const int j0 = q0.x, j1 = m > 1 ? q0.y : j0, j2 = m > 2 ? q0.z : j0, j3 = m > 3 ? q0.w : j0;
         if (B == 4) {
             const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
@@ -576,7 +574,7 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
             if (m > 2) use(a2, j2);
             if (m > 3) use(a3, j3);
         } else {
-            const int4 q1 = L[(size_t)((k >> 2) + (m > 4 ? 1 : 0)) * stride];
+            const int4 q1 = quad((k >> 2) + (m > 4 ? 1 : 0));
             const int j4 = m > 4 ? q1.x : j0, j5 = m > 5 ? q1.y : j0, j6 = m > 6 ? q1.z : j0,
                       j7 = m > 7 ? q1.w : j0;
             const T a0 = load(j0), a1 = load(j1), a2 = load(j2), a3 = load(j3);
@@ -593,6 +591,13 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
     }
 }
 
+template <class T, int B, class Load, class Use>
+__device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) {
+    const int4* __restrict__ L = D.nbr4 + s;
+    const size_t stride = (size_t)D.Nf;
+    for_each_nbr_q<T, B>(D.ncnt[s], [&](int q) { return L[(size_t)q * stride]; }, load, use);
+}
+
 // Neighbour lists fused with the density pass, in two phases so that the expensive
 // per-neighbour work is not paid for every candidate of a divergent wave:
 //   phase 1  list:  j != i, |x_i - x_j|^2 < H^2 (FP32 predicate), candidate order
@@ -603,7 +608,9 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
 // vs[s].w so that the viscosity pass fetches velocity and density together.
 #define NBR_LDS_QUADS 6
-__global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) {
+// Six waves per SIMD (80 VGPRs, no scratch; the LDS staging allows six workgroups of 256
+// per CU) hide more of the gathers' latency than the five the register count gives unasked.
+__global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
     const GridWorld g = D.gw[w];
@@ -666,8 +673,11 @@ __global__ __launch_bounds__(CHUNK) void k_sph_nbr_density(SphDev D, float dt2) 
 
     float sum = xi.w * P.W0, cmp = 0.0f;
     float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
-    for_each_nbr<float4, 8>(
-        D, s, [&](int j) { return D.xs[j]; },
+    // the first NBR_LDS_QUADS quads are read back from LDS
+    for_each_nbr_q<float4, 8>(
+        c < cap ? c : cap,
+        [&](int q) { return q < NBR_LDS_QUADS ? lq[q * bd + t] : out[(size_t)q * Nf]; },
+        [&](int j) { return D.xs[j]; },
         [&](const float4& xj, int) {
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
