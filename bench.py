@@ -27,6 +27,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+PROBE_STEPS = 10  # eager steps with every kernel class timed, before the timed steps
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
 
 
@@ -252,12 +253,13 @@ def main():
     names = [lib.nsph_kernel_name(k).decode() for k in range(nk)]
     ms = (C.c_double * nk)()
     cnt = (C.c_int * nk)()
-    # short eager pass with every class timed, to find the dominant kernel class (graph
-    # replays carry no per-kernel events)
+    # eager pass with every class timed, to find the dominant kernel class (graph replays
+    # carry no per-kernel events); long enough that its iteration counts, which set the
+    # pressure classes' launches per step, are those of the timed steps
     graph_mode = solver.use_graph
     solver.use_graph = False
     N.call("nsph_solver_set_timing", solver._handle, -1)
-    for _ in range(3):
+    for _ in range(PROBE_STEPS):
         solver.step(s0, s1, None, None, dt)
         s0, s1 = s1, s0
     N.call("nsph_solver_timing", solver._handle, ms, cnt)
@@ -290,17 +292,17 @@ def main():
         # graph replays carry no per-kernel events (only the eager finish of a solve whose
         # captured slots were too few does): use the eager all-class pass
         per = per_all
-        roofline_src = "HIP events of a 3-step eager pass (the timed steps replay HIP graphs)"
+        roofline_src = f"HIP events of a {PROBE_STEPS}-step eager pass (the timed steps replay HIP graphs)"
     avg_s = per[dom][0] / max(per[dom][1], 1) / 1e3
     achieved = ab[dom] * nf / avg_s / 1e9
     value = nf * args.steps / el
-    # per-class table from the 3-step all-class pass (event brackets on every launch there)
+    # per-class table from the all-class pass (event brackets on every launch there)
     kern = {k: {"ms_avg": round(per_all[k][0] / max(per_all[k][1], 1), 4),
-                "launches_per_step": per_all[k][1] / 3.0,
+                "launches_per_step": per_all[k][1] / PROBE_STEPS,
                 "GBps": round(ab[k] * nf / (per_all[k][0] / max(per_all[k][1], 1) / 1e3) / 1e9, 1)
                 if ab.get(k, 0.0) > 0 and per_all[k][1] else None}
             for k in names}
-    launches_per_step = {k: per_all[k][1] / 3.0 for k in names}
+    launches_per_step = {k: per_all[k][1] / PROBE_STEPS for k in names}
     step_bytes = sum(ab.get(k, 0.0) * nf * launches_per_step[k] for k in names)
     out = {
         "metric": "particle-updates/s (1M SPH) + % HBM peak",
