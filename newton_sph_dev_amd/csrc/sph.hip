@@ -997,7 +997,8 @@ __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slo
         r.nonfinite = D.flags[2];
         r.pad = 0;
         r.max_err = serr[0];
-        *out = r;
+        *out = r;  // pinned host memory
+        __threadfence_system();
     }
 }
 
@@ -1378,14 +1379,15 @@ struct nsph_solver {
     // device tables
     DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
     DevBuf fid, fworld, oid;
-    DevBuf btable, bxs, ftable, breach, bcend, fcend, rep;
+    DevBuf btable, bxs, ftable, breach, bcend, fcend;
     DevBuf keys, vals;
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags, wmax;
-    int* h_pinned = nullptr;  // [0] act value
+    int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
+    RepDev* d_rep_host = nullptr;  // device address of the pinned RepDev
     int stepped = 0;
     int last_iters_launched = 0;
     int mlp_ready = 0;
@@ -1451,7 +1453,7 @@ static int upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
-                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend, &S->rep,
+                      &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
@@ -1677,7 +1679,6 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     }
 
     NSPH_TRY(buf_ensure(S->breach, off + 1));
-    NSPH_TRY(buf_ensure(S->rep, sizeof(RepDev)));
     hipLaunchKernelGGL(k_bnd_reach, dim3((unsigned)((off + 255) / 256)), dim3(256), 0, st, bp<GridWorld>(S->gw), W,
                        (int)off, bp<int2>(S->btable), bp<unsigned char>(S->breach));
     NSPH_HIP(hipGetLastError());
@@ -1717,20 +1718,18 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
     NSPH_HIP(hipGetLastError());
     NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 128, hipHostMallocDefault));  // [0] act value, [16..] RepDev
+    NSPH_HIP(hipHostGetDevicePointer((void**)&S->d_rep_host, S->h_pinned + 16, 0));
     NSPH_HIP(hipStreamSynchronize(st));
     return NSPH_OK;
 }
 
-// Enqueue the step summary (k_sph_report into S->rep, one copy into pinned host memory)
-// and wait for it: one stream synchronisation per step.  act_slot selects the
-// still-iterating count that is returned with it (-1: none).
+// Enqueue the step summary (k_sph_report writes it straight into pinned host memory, so no
+// copy follows), wait for the stream once and read it.
 static int summarise_step(nsph_solver* S, hipStream_t st, int act_slot, RepDev* out) {
-    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(256), 0, st, make_dev(S), S->W, act_slot, bp<RepDev>(S->rep));
+    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(256), 0, st, make_dev(S), S->W, act_slot, S->d_rep_host);
     NSPH_HIP(hipGetLastError());
-    RepDev* h = reinterpret_cast<RepDev*>(S->h_pinned + 16);
-    NSPH_HIP(hipMemcpyAsync(h, S->rep.ptr, sizeof(RepDev), hipMemcpyDeviceToHost, st));
     NSPH_HIP(hipStreamSynchronize(st));
-    *out = *h;
+    *out = *reinterpret_cast<const RepDev*>(S->h_pinned + 16);
     return NSPH_OK;
 }
 
