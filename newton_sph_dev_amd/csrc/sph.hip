@@ -955,14 +955,22 @@ struct RepDev {
     int act, max_it, sum_it, unconv, overflow, maxnbr, nonfinite, pad;
     double max_err;
 };
-__global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slot, RepDev* __restrict__ out) {
-    __shared__ int smax[256], ssum[256], sun[256], snb[256];
-    __shared__ double serr[256];
+__global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_slot, RepDev* __restrict__ out) {
+    constexpr int NT = 1024;
+    __shared__ int smax[NT], ssum[NT], sun[NT], snb[NT];
+    __shared__ double serr[NT];
     const int t = threadIdx.x;
     int mx = 0, sm = 0, un = 0, nb = 0;
     double er = 0.0;
-    for (int i = t; i < D.nwave; i += 256) nb = max(nb, D.wmax[i]);
-    for (int w = t; w < W; w += 256) {
+    // per-wave list maxima, four per load (nwave = 4 per chunk), several loads in flight
+    const int4* __restrict__ wm = reinterpret_cast<const int4*>(D.wmax);
+    const int n4 = D.nwave >> 2;
+#pragma unroll 4
+    for (int i = t; i < n4; i += NT) {
+        const int4 v = wm[i];
+        nb = max(nb, max(max(v.x, v.y), max(v.z, v.w)));
+    }
+    for (int w = t; w < W; w += NT) {
         if (D.w_nf[w] == 0) continue;
         const int it = D.iters[w];
         mx = max(mx, it);
@@ -976,7 +984,7 @@ __global__ __launch_bounds__(256) void k_sph_report(SphDev D, int W, int act_slo
     snb[t] = nb;
     serr[t] = er;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = NT / 2; off > 0; off >>= 1) {
         if (t < off) {
             smax[t] = max(smax[t], smax[t + off]);
             ssum[t] += ssum[t + off];
@@ -1726,7 +1734,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
 // Enqueue the step summary (k_sph_report writes it straight into pinned host memory, so no
 // copy follows), wait for the stream once and read it.
 static int summarise_step(nsph_solver* S, hipStream_t st, int act_slot, RepDev* out) {
-    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(256), 0, st, make_dev(S), S->W, act_slot, S->d_rep_host);
+    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(1024), 0, st, make_dev(S), S->W, act_slot, S->d_rep_host);
     NSPH_HIP(hipGetLastError());
     NSPH_HIP(hipStreamSynchronize(st));
     *out = *reinterpret_cast<const RepDev*>(S->h_pinned + 16);
