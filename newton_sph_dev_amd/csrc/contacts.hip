@@ -45,6 +45,16 @@ __device__ inline void grid_candidates(const GridView& G, float qx, float qy, fl
     }
 }
 
+// True when a pair is certainly farther apart than touching distance plus cohesion c, so that
+// err = |n| - r_i - r_j > c without taking the square root: |n|^2 > 1.001 (r_i + r_j + c)^2
+// means |n| > 1.0005 (r_i + r_j + c).  With c >= 0 that margin is far above the FP32 rounding
+// of |n|^2, of d = |n| and of d - r_i - r_j (a few units in the last place of r_i + r_j + c),
+// so a pair it rejects is one the exact test rejects; every other pair takes the exact test.
+__device__ inline bool surely_apart(float3 n, float ri, float rj, float c) {
+    const float lim = ri + rj + c;
+    return c >= 0.0f && (n.x * n.x + n.y * n.y + n.z * n.z) > (lim * lim) * 1.001f;
+}
+
 __device__ inline float3 ld3(const float* p, int i) { return make_float3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
 
 // px[s] = (x, r) of point i (NaN for an inactive one), pv[s] = (v, inverse mass or 0), s = inv[i]:
@@ -118,6 +128,7 @@ __global__ void k_xpbd_particle_contacts(GridView G, const int* __restrict__ fla
             if (j == tid) return;
             const float4 pj = G.px[j];
             float3 nrm = make_float3(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z);
+            if (surely_apart(nrm, ri, pj.w, k_cohesion)) return;
             const float d = lenf(nrm);
             const float err = d - ri - pj.w;
             if (err <= k_cohesion) {
@@ -197,6 +208,7 @@ __global__ void k_eval_particle_contact(GridView G, const int* __restrict__ flag
         if (j == tid) return;
         const float4 pj = G.px[j];
         float3 nrm = make_float3(xi.x - pj.x, xi.y - pj.y, xi.z - pj.z);
+        if (surely_apart(nrm, ri, pj.w, k_cohesion)) return;
         const float d = lenf(nrm);
         const float err = d - ri - pj.w;
         if (err <= k_cohesion) {
