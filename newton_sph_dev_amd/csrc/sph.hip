@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include <vector>
@@ -952,10 +953,10 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
 // counts, worlds not converged, max density error, the neighbour and finiteness flags, and
 // (act_slot >= 0) the still-iterating count act[act_slot].
 struct RepDev {
-    int act, max_it, sum_it, unconv, overflow, maxnbr, nonfinite, pad;
+    int act, max_it, sum_it, unconv, overflow, maxnbr, nonfinite, seq;  // seq: written last
     double max_err;
 };
-__global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_slot, RepDev* __restrict__ out) {
+__global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_slot, int seq, RepDev* __restrict__ out) {
     constexpr int NT = 1024;
     __shared__ int smax[NT], ssum[NT], sun[NT], snb[NT];
     __shared__ double serr[NT];
@@ -1003,9 +1004,12 @@ __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_sl
         r.overflow = D.flags[0];
         r.maxnbr = snb[0];
         r.nonfinite = D.flags[2];
-        r.pad = 0;
+        r.seq = seq - 1;
         r.max_err = serr[0];
         *out = r;  // pinned host memory
+        __threadfence_system();
+        // the host waits for this sequence number; the fields above are visible before it
+        *reinterpret_cast<volatile int*>(&out->seq) = seq;
         __threadfence_system();
     }
 }
@@ -1396,6 +1400,7 @@ struct nsph_solver {
     DevBuf partial, iters, err, conv, act, flags, wmax;
     int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
     RepDev* d_rep_host = nullptr;  // device address of the pinned RepDev
+    int rep_seq = 0;               // sequence number of the last step summary
     int stepped = 0;
     int last_iters_launched = 0;
     int mlp_ready = 0;
@@ -1734,10 +1739,34 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
 // Enqueue the step summary (k_sph_report writes it straight into pinned host memory, so no
 // copy follows), wait for the stream once and read it.
 static int summarise_step(nsph_solver* S, hipStream_t st, int act_slot, RepDev* out) {
-    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(1024), 0, st, make_dev(S), S->W, act_slot, S->d_rep_host);
+    const int want = S->rep_seq = (S->rep_seq % 1000000000) + 1;
+    hipLaunchKernelGGL(k_sph_report, dim3(1), dim3(1024), 0, st, make_dev(S), S->W, act_slot, want,
+                       S->d_rep_host);
     NSPH_HIP(hipGetLastError());
-    NSPH_HIP(hipStreamSynchronize(st));
-    *out = *reinterpret_cast<const RepDev*>(S->h_pinned + 16);
+    // Wait for the summary's sequence number without letting the thread sleep (a blocking
+    // stream wait can add an interrupt wake-up to every step).  The stream is asked for
+    // errors every 1024 polls; if it went idle without the number, synchronise to report why.
+    volatile RepDev* h = reinterpret_cast<volatile RepDev*>(S->h_pinned + 16);
+    for (long k = 1; h->seq != want; ++k) {
+        if ((k & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(st);
+            if (e == hipErrorNotReady) continue;
+            NSPH_HIP(e);
+            if (h->seq == want) break;
+            NSPH_HIP(hipStreamSynchronize(st));
+            NSPH_CHECK(h->seq == want, NSPH_ERR_HIP, "step summary: the stream finished without it");
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    out->act = h->act;
+    out->max_it = h->max_it;
+    out->sum_it = h->sum_it;
+    out->unconv = h->unconv;
+    out->overflow = h->overflow;
+    out->maxnbr = h->maxnbr;
+    out->nonfinite = h->nonfinite;
+    out->seq = h->seq;
+    out->max_err = h->max_err;
     return NSPH_OK;
 }
 
