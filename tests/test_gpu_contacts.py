@@ -216,3 +216,68 @@ def test_particle_particle_friction_with_relative_motion():
     slip_with_friction = run(1.0)
     assert slip_no_friction > 0.2
     assert slip_with_friction < slip_no_friction * 0.95
+
+
+def _threshold_pairs(radius=0.05, cohesion=0.01, seed=7):
+    """Isolated pairs (1 m apart) at distances within a few units in the last place of the
+    contact limit 2 r + cohesion, and at +-0.04 % and +-0.1 % of it, in random directions.
+    Returns the model inputs and, per pair, whether the reference's FP32 test accepts it:
+    n = x_a - x_b, d = sqrt((nx nx + ny ny) + nz nz), err = (d - r) - r, accept when
+    err <= cohesion (float32 throughout, as the kernels compute it without contraction)."""
+    rng = np.random.default_rng(seed)
+    lim = 2 * radius + cohesion
+    rel = np.concatenate([np.arange(-24, 25) * 2.0 ** -24, [-1e-3, -4e-4, 4e-4, 1e-3]])
+    k = 10 * 10 * 10
+    centres = np.stack(np.meshgrid(*[np.arange(10)] * 3, indexing="ij"), -1).reshape(-1, 3) * 1.0 + 0.5
+    u = rng.normal(size=(k, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    dist = lim * (1.0 + rel[np.arange(k) % len(rel)])
+    xa = centres.astype(np.float32)
+    xb = (centres + u * dist[:, None]).astype(np.float32)
+    f32 = np.float32
+    nv = xa - xb
+    d = np.sqrt((nv[:, 0] * nv[:, 0] + nv[:, 1] * nv[:, 1]) + nv[:, 2] * nv[:, 2])
+    err = (d - f32(radius)) - f32(radius)
+    accept = err <= f32(cohesion)
+    assert np.all(err != 0)
+    x = np.empty((2 * k, 3), np.float32)
+    x[0::2], x[1::2] = xa, xb
+    return x, accept
+
+
+@pytest.mark.parametrize("kernel", ["xpbd", "penalty"])
+def test_contact_threshold_pairs_decided_exactly(kernel):
+    """Pairs within a few ulps of touching distance plus cohesion: a pair gets a correction or
+    a force exactly when the reference's FP32 test accepts it, so the kernels' early
+    squared-distance rejection never drops a touching pair."""
+    radius, cohesion = 0.05, 0.01
+    x, accept = _threshold_pairs(radius, cohesion)
+    n = len(x)
+    _print(f"{kernel}: {n // 2} isolated pairs around the contact limit, {int(accept.sum())} touching")
+    assert 0 < accept.sum() < len(accept)
+    b = ModelBuilder(up_axis="Y")
+    b.add_particles(pos=x.astype(np.float64).tolist(), vel=[[0.0, 0.0, 0.0]] * n, mass=[1.0] * n,
+                    radius=[radius] * n)
+    m = b.finalize(DEV)
+    m.particle_cohesion = cohesion
+    s = m.state()
+    s.particle_q.copy_(torch.from_numpy(x).to(DEV))
+    grid = m.particle_grid
+    grid.build(s.particle_q, radius=2 * radius + cohesion)
+    if kernel == "xpbd":
+        out = torch.zeros((n, 3), device=DEV)
+        N.call("nsph_xpbd_particle_contacts", grid._handle, n, N.ptr(s.particle_q), N.ptr(s.particle_qd),
+               N.ptr(m.particle_inv_mass), N.ptr(m.particle_radius), N.ptr(m.particle_flags), 0.4, cohesion,
+               radius, 1e-3, 1.0, N.ptr(out))
+    else:
+        out = s.particle_f
+        out.zero_()
+        N.call("nsph_eval_particle_contact", grid._handle, n, N.ptr(s.particle_q), N.ptr(s.particle_qd),
+               N.ptr(m.particle_radius), N.ptr(m.particle_flags), 1.0e3, 0.0, 0.0, 0.4, cohesion, radius,
+               N.ptr(out))
+    torch.cuda.synchronize()
+    got = np.linalg.norm(out.cpu().numpy().astype(np.float64), axis=1)
+    touched = (got[0::2] > 0) & (got[1::2] > 0)
+    untouched = (got[0::2] == 0) & (got[1::2] == 0)
+    assert np.all(touched | untouched)
+    assert np.array_equal(touched, accept)
