@@ -10,7 +10,7 @@
 //   dH1[s][k] = (bf16(dH2) W2^T)[s][k] [h1 > 0]   (MFMA)      db1 += dH1 (FP32)
 //   dW1      += bf16(x)^T bf16(dH1)          (MFMA, K = 32 samples)
 // Each wave keeps its partial sums in registers and writes them once to a parameter-major
-// partial buffer; k_mlp_reduce sums the waves in index order, so gradients are bitwise
+// partial buffer; k_mlp_reduce sums them in a fixed order, so gradients are bitwise
 // repeatable.  Adam (FP32) then refreshes the BF16 copies.
 
 #include <math.h>
@@ -239,13 +239,19 @@ __global__ __launch_bounds__(64) void k_mlp_grad(MlpDev M, const float* __restri
     }
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ partial, int G, float* __restrict__ grad) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > MLP_NPARAM) return;
+// Sum of the G partials of each gradient entry: one wave per entry; lane l adds partials
+// l, l + 64, ... in order (coalesced across the wave), then the wave adds its lanes in a
+// fixed butterfly, so the result is bitwise repeatable.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ partial, int G, float* __restrict__ grad) {
+    const int p = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int l = (int)(threadIdx.x & 63u);
+    if (p > MLP_NPARAM) return;  // whole waves
     const float* row = partial + (size_t)p * G;
     float acc = 0.f;
-    for (int w = 0; w < G; ++w) acc += row[w];
-    grad[p] = acc;
+    for (int w = l; w < G; w += 64) acc += row[w];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    if (l == 0) grad[p] = acc;
 }
 
 __global__ void k_mlp_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
@@ -393,7 +399,7 @@ int nsph_mlp_grad(nsph_mlp* m, const float* d_x, const float* d_t, int n, float*
     const int G = m->partial_waves;
     hipLaunchKernelGGL(k_mlp_grad, dim3(G), dim3(64), 0, st, mlp_dev(m), d_x, d_t, n, 1.0f / (float)n,
                        bp<float>(m->partial), G);
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 1 + 255) / 256), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 1 + 3) / 4), dim3(256), 0, st,
                        bp<float>(m->partial), G, bp<float>(m->grad));
     NSPH_HIP(hipGetLastError());
     if (d_grad)
