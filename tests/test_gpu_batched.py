@@ -145,3 +145,57 @@ def test_autotune_picks_a_config(tmp_path):
     assert (tmp_path / "tune.json").exists()
     again = autotune.tune(s, a, c, scenes.DT, cache=tmp_path / "tune.json")
     assert again["gather_block"] == best["gather_block"]
+
+
+def _tiny_batch():
+    """Worlds of 0, 1, 3 and 5 fluid particles and two ordinary blocks (120 and 210), each in
+    its own walled box: partial waves and quads, lanes past a world's end, and a world with
+    no fluid at all."""
+    from newton_sph_dev_amd import ModelBuilder
+    from newton_sph_dev_amd.sim import ParticleFlags
+
+    sp = 0.02
+    rng = np.random.default_rng(5)
+    b = ModelBuilder()
+    box = (10 * sp, 8 * sp, 8 * sp)
+    for nf in (0, 1, 120, 3, 5, 210):
+        b.begin_world()
+        if nf in (120, 210):
+            dims = (6, 5, 4) if nf == 120 else (7, 6, 5)
+            b.add_particle_grid(pos=(0.5 * sp, 0.5 * sp, 0.5 * sp), rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0),
+                                dim_x=dims[0], dim_y=dims[1], dim_z=dims[2], cell_x=sp, cell_y=sp, cell_z=sp,
+                                mass=scenes.REST_DENSITY * sp ** 3, jitter=0.1 * sp, radius_mean=0.5 * sp)
+        elif nf:
+            pos = 0.5 * np.array(box) + rng.uniform(-1.5 * sp, 1.5 * sp, (nf, 3))
+            b.add_particles(pos, np.zeros_like(pos), np.full(nf, scenes.REST_DENSITY * sp ** 3),
+                            np.full(nf, 0.5 * sp), np.full(nf, int(ParticleFlags.ACTIVE)))
+        bp = scenes.box_boundary_points(sp, box, 2)
+        b.add_particles(bp, np.zeros_like(bp), np.zeros(bp.shape[0]), np.full(bp.shape[0], 0.5 * sp),
+                        np.full(bp.shape[0], int(ParticleFlags.ACTIVE)))
+        b.end_world()
+    return b, np.full(6, 2 * sp)
+
+
+def test_tiny_and_empty_worlds_in_a_batch():
+    """A batch with worlds of 0, 1, 3 and 5 fluid particles: one step equals the FP32
+    restatement bit for bit, and three steps of every world equal that world run alone."""
+    _print("6 batched worlds with 0, 1, 120, 3, 5 and 210 fluid particles")
+    b, H = _tiny_batch()
+    m = b.finalize("cuda:0")
+    s = SolverSPH(m, H)
+    a, c = m.state(), m.state()
+    s.step(a, c, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    pb = O.problem_from_model(m, s)
+    r = O.step(pb, pb.q0, pb.qd0, precision="f32")
+    assert np.array_equal(r["q"], c.particle_q.cpu().numpy())
+    assert np.array_equal(r["qd"], c.particle_qd.cpu().numpy())
+    assert np.array_equal(r["iters"], s.world_stats()["iterations"])
+    s3, st, iters = _run(m, H, 3)
+    q = st.particle_q.cpu().numpy()
+    world = m.particle_world.cpu().numpy()
+    for w in range(len(H)):
+        mw = scenes.extract_world(m, w).finalize("cuda:0")
+        sw, stw, itw = _run(mw, H[w], 3)
+        assert np.array_equal(stw.particle_q.cpu().numpy(), q[world == w]), w
+        assert [int(i[0]) for i in itw] == [int(i[w]) for i in iters], w
