@@ -267,3 +267,25 @@ def test_dam_break_32k_one_step():
     f_rel = np.abs(f1[fid] - r64["f"][fid]).max() / np.abs(r64["f"][fid]).max()
     _print(f"  iters={rep['max_iterations']} rho_rel={rho_rel:.2e} f_rel={f_rel:.2e} nbr_max={rep['max_neighbor_count']}")
     assert rho_rel <= RHO_TOL and f_rel <= F_TOL
+
+
+def test_list_overflow_is_reported():
+    """max_neighbors below the real list lengths: the step raises CapacityError, and its
+    report carries the overflow flag and the true (uncapped) largest count."""
+    from newton_sph_dev_amd import _native as N
+
+    cfg = scenes.small_dam_break(12, 10, 14)
+    _print("dam break 12x10x14 with max_neighbors=8")
+    m, s = _solver_and_model(cfg, max_neighbors=8)
+    st0, st1 = m.state(), m.state()
+    with pytest.raises(N.CapacityError):
+        s.step(st0, st1, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    rep = s.last_report
+    assert rep["neighbor_overflow"] == 1
+    pb = O.problem_from_model(m, s)
+    fluid = ((pb.flags & 1) != 0) & (pb.inv_mass > 0)
+    fid = np.nonzero(fluid)[0]
+    H2 = np.float32(cfg.support * cfg.support)
+    bc, _ = O.neighbors_brute(pb.q0[fid], None, H2, cap=96)
+    assert rep["max_neighbor_count"] == int(bc.max()) > 8
