@@ -289,3 +289,38 @@ def test_list_overflow_is_reported():
     H2 = np.float32(cfg.support * cfg.support)
     bc, _ = O.neighbors_brute(pb.q0[fid], None, H2, cap=96)
     assert rep["max_neighbor_count"] == int(bc.max()) > 8
+
+
+def test_hashgrid_capacity_overflow():
+    """Lists longer than max_neighbors: the call returns the capacity error, the counts are
+    the true lengths and the filled slots are the first max_neighbors of the full list in
+    candidate order (grid neighbours and arbitrary query points)."""
+    from newton_sph_dev_amd import _native as N
+
+    _print("hashgrid: 4000 points, radius 0.06, capacity below the longest list")
+    rng = np.random.default_rng(29)
+    pts = (rng.random((4000, 3)) * 0.6).astype(np.float32)
+    g = HashGrid(16, 16, 16, device="cuda:0")
+    g.build(torch.from_numpy(pts).cuda(), 0.06)
+    full_c, full_i = (t.cpu().numpy() for t in g.neighbors(0.06, max_neighbors=128))
+    bc, bi = O.neighbors_brute(pts, None, np.float32(0.06) * np.float32(0.06), cap=128)
+    assert np.array_equal(full_c, bc) and int(bc.max()) < 128
+    cap = int(bc.max()) - 5
+    qp = torch.from_numpy((rng.random((700, 3)) * 0.6).astype(np.float32)).cuda()
+    fq_c, fq_i = (t.cpu().numpy() for t in g.query(qp, 0.06, max_neighbors=128))
+    assert int(fq_c.max()) < 128
+    for name, args, fc, fi, n in (
+            ("nsph_grid_neighbors", (None, 0.06), full_c, full_i, 4000),
+            ("nsph_grid_query", (N.ptr(qp), 700, 0.06), fq_c, fq_i, 700)):
+        counts = torch.empty(n, dtype=torch.int32, device="cuda:0")
+        idx = torch.full((cap, n), -1, dtype=torch.int32, device="cuda:0")
+        with pytest.raises(N.CapacityError):
+            N.call(name, g._handle, *args, cap, N.ptr(counts), N.ptr(idx))
+        torch.cuda.synchronize()
+        c, i = counts.cpu().numpy(), idx.cpu().numpy()
+        assert np.array_equal(c, fc)
+        assert (c > cap).any()
+        for k in range(n):
+            m = min(int(c[k]), cap)
+            assert np.array_equal(i[:m, k], fi[:m, k])
+            assert (i[m:, k] == -1).all()
