@@ -10,7 +10,7 @@ from p0 = max(0.5 p_prev + clamp(y, +-8) p_unit, 0) for particles with at least
 GATE_NEIGHBOURS fluid + boundary neighbours (0.5 p_prev elsewhere: at the free surface and
 in splashes p_unit is large and a network output would become a pressure spike), and always
 finishes with the plain Jacobi iterations to the same tolerance, so the plain solver stays
-the fallback and the judge of convergence.
+the fallback and what decides convergence.
 
 Training data are solver snapshots: after each step of seeded rollouts of the dam-break
 generator with random block extents, the features and the target (converged pressure
