@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <cmath>
 
 #include <vector>
@@ -1730,7 +1731,12 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((nf1 + 255) / 256)), dim3(256), 0, st, bp<float>(S->pprev), nf1);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->flags), (size_t)4, 0);
     NSPH_HIP(hipGetLastError());
-    NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 128, hipHostMallocDefault));  // [0] act value, [16..] RepDev
+    // [0] act value, [16..] RepDev.  Coherent (fine-grained) so that the report kernel's
+    // system-scope stores reach the host while the stream is still running.
+    NSPH_HIP(hipHostMalloc((void**)&S->h_pinned, 128, hipHostMallocMapped | hipHostMallocCoherent));
+    // The block may be one a destroyed solver used: clear its last sequence number, or the
+    // first summary wait could match it and return that solver's report.
+    std::memset(S->h_pinned, 0, 128);
     NSPH_HIP(hipHostGetDevicePointer((void**)&S->d_rep_host, S->h_pinned + 16, 0));
     NSPH_HIP(hipStreamSynchronize(st));
     return NSPH_OK;

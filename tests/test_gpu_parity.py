@@ -289,6 +289,17 @@ def test_list_overflow_is_reported():
     H2 = np.float32(cfg.support * cfg.support)
     bc, _ = O.neighbors_brute(pb.q0[fid], None, H2, cap=96)
     assert rep["max_neighbor_count"] == int(bc.max()) > 8
+    # A new solver after this one-step solver is gone: its first report is its own (the
+    # pinned block that carries step reports may be recycled with the old report in it).
+    del s
+    m2, s2 = _solver_and_model(cfg)
+    st0, st1 = m2.state(), m2.state()
+    s2.step(st0, st1, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    rep2 = s2.last_report
+    assert rep2["neighbor_overflow"] == 0
+    assert rep2["max_neighbor_count"] == int(bc.max())
+    assert rep2["max_iterations"] >= 2
 
 
 def test_hashgrid_capacity_overflow():
