@@ -207,3 +207,11 @@ struct nsph_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
 };
+
+// Before an object's buffers are freed: make its device current and wait for the work
+// queued on its context's stream, which may still read them.
+static inline hipError_t nsph_ctx_drain(const nsph_ctx* c) {
+    if (!c) return hipSuccess;
+    const hipError_t e = hipSetDevice(c->device);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+}

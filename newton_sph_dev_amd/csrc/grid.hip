@@ -363,6 +363,7 @@ int nsph_grid_query(const nsph_grid* g, const float* d_q, int nq, float radius, 
 
 int nsph_grid_destroy(nsph_grid* g) {
     if (!g) return NSPH_OK;
+    const hipError_t e = nsph_ctx_drain(g->ctx);
     buf_free(g->keys);
     buf_free(g->vals);
     buf_free(g->xs);
@@ -373,6 +374,7 @@ int nsph_grid_destroy(nsph_grid* g) {
     buf_free(g->inv);
     radix_free(g->rs);
     delete g;
+    NSPH_HIP(e);
     return NSPH_OK;
 }
 

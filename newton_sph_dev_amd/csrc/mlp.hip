@@ -356,10 +356,12 @@ int nsph_mlp_create(nsph_ctx* ctx, nsph_mlp** out) {
 
 int nsph_mlp_destroy(nsph_mlp* m) {
     if (!m) return NSPH_OK;
+    const hipError_t e = nsph_ctx_drain(m->ctx);
     buf_free(m->P); buf_free(m->grad); buf_free(m->m); buf_free(m->v);
     buf_free(m->W1b); buf_free(m->W2b); buf_free(m->w3b); buf_free(m->partial);
     if (m->h_loss) (void)hipHostFree(m->h_loss);
     delete m;
+    NSPH_HIP(e);
     return NSPH_OK;
 }
 

@@ -2207,8 +2207,10 @@ int nsph_solver_iterations_launched(const nsph_solver* S, int* n) {
 
 int nsph_solver_destroy(nsph_solver* S) {
     if (!S) return NSPH_OK;
+    const hipError_t e = nsph_ctx_drain(S->ctx);
     solver_free(S);
     delete S;
+    NSPH_HIP(e);
     return NSPH_OK;
 }
 
