@@ -70,22 +70,36 @@ def test_batched_worlds_equal_fp32_restatement():
 
 @pytest.mark.parametrize("slots", [None, 1])
 def test_graph_replay_equals_eager(slots):
+    """Two captured steps (a -> b, b -> a) replayed 12 times in ping-pong; after every replay
+    the state equals the same eager step's bit for bit, with a synchronisation and eager
+    torch work on the output between replays."""
     cfg = scenes.small_dam_break(14, 12, 16)
-    _print(f"graph replay (iter_slots={slots}) vs eager, 12 steps ping-pong")
+    _print(f"graph replay (iter_slots={slots}) vs eager, 12 steps ping-pong, checked per replay")
     m = scenes.build_dam_break(cfg).finalize("cuda:0")
-    se, ste, _ = _run(m, cfg.support, 12, warm_start="prev")
+    se = SolverSPH(m, cfg.support, warm_start="prev")
+    x, y = m.state(), m.state()
+    ref = []
+    for _ in range(12):
+        se.step(x, y, None, None, scenes.DT)
+        x, y = y, x
+        ref.append((x.particle_q.cpu().numpy(), x.particle_qd.cpu().numpy()))
     sg = SolverSPH(m, cfg.support, warm_start="prev")
     a, b = m.state(), m.state()
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         g_ab = sg.capture(a, b, scenes.DT, iter_slots=slots)
         g_ba = sg.capture(b, a, scenes.DT, iter_slots=slots)
-        for k in range(12):
+    for k in range(12):
+        with torch.cuda.stream(stream):
             sg.replay(g_ab if k % 2 == 0 else g_ba)
-    torch.cuda.synchronize()
-    assert np.array_equal(a.particle_q.cpu().numpy(), ste.particle_q.cpu().numpy())
-    assert np.array_equal(a.particle_qd.cpu().numpy(), ste.particle_qd.cpu().numpy())
-    assert sg.last_report["unconverged_worlds"] == 0
+        torch.cuda.synchronize()
+        out = b if k % 2 == 0 else a
+        assert np.array_equal(out.particle_q.cpu().numpy(), ref[k][0]), f"replay {k}"
+        assert np.array_equal(out.particle_qd.cpu().numpy(), ref[k][1]), f"replay {k}"
+        assert sg.last_report["unconverged_worlds"] == 0
+        scratch = out.particle_q.clone().mul_(2.0).sum(dim=0)  # eager kernels between replays
+        torch.cuda.synchronize()
+        del scratch
 
 
 def test_launch_configs_do_not_change_results():
