@@ -167,3 +167,18 @@ def test_trained_mlp_reduces_iterations_vs_zero_warm_start():
     _print(f"  zero {res['zero']['mean_iterations']:.2f} it, mlp {res['mlp']['mean_iterations']:.2f} it")
     assert res["mlp"]["max_density_error"] <= 1e-3
     assert res["mlp"]["mean_iterations"] < res["zero"]["mean_iterations"]
+
+
+def test_mlp_warm_start_in_graph_mode_equals_eager():
+    """The learned warm start in captured (HIP graph) steps over a batch of worlds: iteration
+    counts and states equal the eager steps bit for bit, step after step."""
+    cfg = scenes.small_batch(10, seed=5)
+    _print("random MLP warm start, 10 worlds, graph mode vs eager, 6 steps")
+    b, H = scenes.build_batched(cfg)
+    m = b.finalize("cuda:0")
+    mlp = WarmStartMLP("cuda:0", seed=3)
+    _, a0, it0 = _rollout(m, H, "mlp", 6, mlp=mlp)
+    _, a1, it1 = _rollout(m, H, "mlp", 6, mlp=mlp, use_graph=True, graph_slots=2)
+    assert it0 == it1
+    assert np.array_equal(a0.particle_q.cpu().numpy(), a1.particle_q.cpu().numpy())
+    assert np.array_equal(a0.particle_qd.cpu().numpy(), a1.particle_qd.cpu().numpy())
