@@ -241,6 +241,59 @@ __device__ inline void mark_cell(const uint32_t* __restrict__ skeys, int s, int 
     }
 }
 
+// mark_cell for a whole wave of consecutive sorted indices s (every lane calls it; lanes with
+// s >= n only pass values along).  The neighbours' keys come from the adjacent lanes, and the
+// per-slab counts from the cell's first lane in the wave come from a segmented inclusive scan;
+// only a cell that began before the wave walks back through the keys in memory, from the
+// wave's first index, with the same stop as mark_cell, so the table words are the same.
+__device__ inline void mark_cell_wave(const uint32_t* __restrict__ skeys, int s, int n,
+                                      int2* __restrict__ table, int* __restrict__ cend) {
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    const uint32_t key = valid ? skeys[s] : 0u;
+    const uint32_t cell = key >> NSPH_XSUB_BITS;
+    uint32_t kprev = (uint32_t)__shfl_up((int)key, 1, 64);
+    uint32_t knext = (uint32_t)__shfl_down((int)key, 1, 64);
+    if (lane == 0 && valid && s > 0) kprev = skeys[s - 1];
+    if (lane == 63 && valid && s + 1 < n) knext = skeys[s + 1];
+    const bool start = s == 0 || lane == 0 || (kprev >> NSPH_XSUB_BITS) != cell;
+    const bool before = lane == 0 && s > 0 && (kprev >> NSPH_XSUB_BITS) == cell;  // cell began earlier
+    const bool end = s == n - 1 || (knext >> NSPH_XSUB_BITS) != cell;
+    int seg = start ? lane : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(seg, d, 64);
+        seg = o > seg ? o : seg;
+    }
+    uint32_t acc = 1u << (8u * (key & (NSPH_XSUB - 1)));
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t a = (uint32_t)__shfl_up((int)acc, d, 64);
+        if (lane - d >= seg) acc += a;
+    }
+    const bool walk = __shfl((int)before, 0, 64) != 0 && seg == 0;
+    if (!valid) return;
+    if (start && !before) table[cell].x = s;
+    if (end) {
+        cend[cell] = s + 1;
+        const int stop = s - (NSPH_SLAB_SAT - 1) > 0 ? s - (NSPH_SLAB_SAT - 1) : 0;
+        int j = s - (lane - seg);
+        if (walk) {
+            while (j > stop && (skeys[j - 1] >> NSPH_XSUB_BITS) == cell) {
+                --j;
+                acc += 1u << (8u * (skeys[j] & (NSPH_XSUB - 1)));
+            }
+        }
+        const int cnt = s + 1 - j;
+        uint32_t w = (uint32_t)NSPH_SLAB_SAT << 24;
+        if (cnt < NSPH_SLAB_SAT) {
+            const uint32_t p1 = acc & 0xffu, p2 = p1 + ((acc >> 8) & 0xffu), p3 = p2 + ((acc >> 16) & 0xffu);
+            w = p1 | (p2 << 8) | (p3 << 16) | ((uint32_t)cnt << 24);
+        }
+        table[cell].y = (int)w;
+    }
+}
+
 __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restrict__ qd,
                               const float* __restrict__ mass, const int* __restrict__ fid,
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
@@ -248,15 +301,16 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
                               float4* __restrict__ vs, float* __restrict__ pws,
                               int2* __restrict__ ftable, int* __restrict__ fcend,
                               int* __restrict__ inv) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int k = (int)perm[s];
-    const int id = fid[k];
-    inv[k] = s;
-    xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
-    vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
-    pws[s] = pprev[k];
-    mark_cell(skeys, s, n, ftable, fcend);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;  // blockDim.x: a multiple of 64
+    if (s < n) {
+        const int k = (int)perm[s];
+        const int id = fid[k];
+        inv[k] = s;
+        xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
+        vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
+        pws[s] = pprev[k];
+    }
+    mark_cell_wave(skeys, s, n, ftable, fcend);
 }
 
 // Start of a step, one thread per world and per fluid particle (fluid index k):
