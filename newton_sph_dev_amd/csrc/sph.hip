@@ -1959,9 +1959,11 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
 }
 
 // guard: nullptr, or the still-iterating count that must be 0 for the fluid part to run.
+// boundary: also integrate the non-fluid particles (unguarded: they do not depend on the
+// pressure solve, so a step that enqueues its end again skips them the second time).
 static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
                             float* q_out, float* qd_out, float* f_out, float dt,
-                            const int* guard = nullptr) {
+                            const int* guard = nullptr, bool boundary = true) {
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
     if (S->Nf > 0) {
@@ -1971,7 +1973,7 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, c
                                                S->d_inv_mass, S->Nf, dt, S->v_max, q_out, qd_out, f_out,
                                                bp<float>(S->pprev)));
     }
-    if (S->No > 0) {
+    if (S->No > 0 && boundary) {
         TIMED(KT_PASSTHROUGH,
               hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
                                  bp<int>(S->oid), S->No, q_in, qd_in, S->d_inv_mass, S->d_flags,
@@ -2021,11 +2023,11 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
     // on the device per world, so the results do not depend on this schedule.
     const int first = std::min(std::max(S->min_iters, 1), S->max_iters);
     NSPH_TRY(enqueue_iterations(S, st, dt, 0, first, 0, &launched));
-    for (;;) {
+    for (bool boundary = true;; boundary = false) {
         const bool last = launched >= S->max_iters;
         const int slot = (launched - 1) & 1;
         NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt,
-                                  last ? nullptr : bp<int>(S->act) + slot));
+                                  last ? nullptr : bp<int>(S->act) + slot, boundary));
         RepDev d;
         NSPH_TRY(summarise_step(S, st, last ? -1 : slot, &d));
         if (last || d.act == 0) {
