@@ -335,3 +335,22 @@ def test_hashgrid_capacity_overflow():
             m = min(int(c[k]), cap)
             assert np.array_equal(i[:m, k], fi[:m, k])
             assert (i[m:, k] == -1).all()
+
+
+def test_hashgrid_active_filter():
+    """neighbors(flags=...): inactive points get count 0 and appear in no list; active lists
+    equal brute force over the active points (FP32 predicate)."""
+    _print("hashgrid: 5000 points, 30 % inactive, radius 0.05")
+    rng = np.random.default_rng(41)
+    pts = (rng.random((5000, 3)) * 0.7).astype(np.float32)
+    flags = np.where(rng.random(5000) < 0.3, 0, 1).astype(np.int32)
+    g = HashGrid(16, 16, 16, device="cuda:0")
+    g.build(torch.from_numpy(pts).cuda(), 0.05)
+    counts, idx = g.neighbors(0.05, flags=torch.from_numpy(flags).cuda(), max_neighbors=96)
+    counts, idx = counts.cpu().numpy(), idx.cpu().numpy()
+    bc, bi = O.neighbors_brute(pts, flags, np.float32(0.05) * np.float32(0.05), cap=96)
+    assert np.array_equal(counts, bc)
+    assert (counts[flags == 0] == 0).all()
+    assert _lists_as_sets(counts, idx, True) == _lists_as_sets(bc, bi, False)
+    used = idx[idx >= 0]
+    assert (flags[used] == 1).all()
