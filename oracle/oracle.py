@@ -148,9 +148,11 @@ def problem_from_model(model, solver_like) -> Problem:
 
 
 def step(pb: Problem, q_in, qd_in, pprev=None, dt: float = 1e-4, precision: str = "f64",
-         forced_iters=None, nthreads: int = 0) -> dict:
+         forced_iters=None, nthreads: int = 0, f_in=None) -> dict:
     """One SPH step on the CPU. Returns q, qd, f, rho, p (particle order), neighbour counts,
-    per-world iterations and errors, the updated pprev (fluid-list order) and the status."""
+    per-world iterations and errors, the updated pprev (fluid-list order) and the status.
+    f_in: optional caller force per particle (N x 3), added to the SPH force before the
+    reference integrator (solver.py:24-46 integrates the total particle_f)."""
     L = lib()
     s = pb._struct()
     n, W = pb.n, pb.n_worlds
@@ -168,7 +170,8 @@ def step(pb: Problem, q_in, qd_in, pprev=None, dt: float = 1e-4, precision: str 
     fi = None if forced_iters is None else np.ascontiguousarray(forced_iters, dtype=np.int32)
     nt = nthreads if nthreads > 0 else L.orc_num_threads()
     fn = L.orc_step_f32 if precision == "f32" else L.orc_step_f64
-    st = fn(C.byref(s), _p(q_in), _p(qd_in), _p(pp), C.c_float(dt), _p(fi), _p(out["q"]),
+    fin = None if f_in is None else np.ascontiguousarray(f_in, dtype=np.float32).reshape(-1)
+    st = fn(C.byref(s), _p(q_in), _p(qd_in), _p(fin), _p(pp), C.c_float(dt), _p(fi), _p(out["q"]),
             _p(out["qd"]), _p(out["f"]), _p(out["rho"]), _p(out["p"]), _p(out["nbr_count"]),
             _p(out["iters"]), _p(out["err"]), C.c_int(nt))
     out["status"] = st
@@ -221,6 +224,22 @@ def kernel_f64(H: float, r2: float) -> tuple[float, float]:
     W, gf = C.c_double(), C.c_double()
     L.orc_kernel_f64(C.c_double(H), C.c_double(r2), C.byref(W), C.byref(gf))
     return W.value, gf.value
+
+
+def viscosity(pb: Problem, q_in, qd_in, precision: str = "f64") -> tuple[np.ndarray, np.ndarray]:
+    """Viscous acceleration a_visc (N x 3) and density (N) of one state, particle order, from
+    the same density and viscosity passes as the step (0 for non-fluid particles)."""
+    L = lib()
+    s = pb._struct()
+    q_in = np.ascontiguousarray(q_in, dtype=np.float32)
+    qd_in = np.ascontiguousarray(qd_in, dtype=np.float32)
+    av = np.zeros((pb.n, 3), np.float64)
+    rho = np.zeros(pb.n, np.float64)
+    fn = L.orc_viscosity_f32 if precision == "f32" else L.orc_viscosity_f64
+    st = fn(C.byref(s), _p(q_in), _p(qd_in), _p(av), _p(rho))
+    if st:
+        raise RuntimeError(f"orc_viscosity failed ({st})")
+    return av, rho
 
 
 def operator_f64(pb: Problem, q_in, qd_in, p_fluid, dt: float = 1e-4) -> dict:

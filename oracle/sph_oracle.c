@@ -6,6 +6,7 @@
  *
  * Exports (see oracle/oracle.py for the Python wrapper):
  *   orc_step_f64   FP64 oracle of one step (neighbour sets from the FP32 predicate)
+ *   orc_viscosity_f64 / _f32   density and viscous acceleration of one state (test hook)
  *   orc_step_f32   same-order FP32 restatement of the gfx950 kernels (bit-exact target;
  *                  also the CPU baseline, OpenMP over particles)
  *   orc_neighbors_brute   O(N^2) neighbour sets under the FP32 predicate
@@ -333,22 +334,25 @@ static double orc_world_reduce(const double* e, int start, int n) {
 /* ---- exported -------------------------------------------------------------------------- */
 
 /* One SPH step (same contract as nsph_solver_step plus dumps).
+ *   f_in: NULL, or the caller's force per particle (state_in.particle_f, N x 3), added to the
+ *         SPH force before integration (integrate_particles, solver.py:24-46, integrates the
+ *         total force of particle_f)
  *   pprev: previous-step pressure in fluid-list order (ascending fluid id), in/out
  *   forced_iters: NULL, or per world the exact number of Jacobi updates (-1 = stop test)
  *   returns 0, 3 (neighbour overflow) or 6 (a world hit max_iters). */
-int orc_step_f32(const orc_problem* pb, const float* q_in, const float* qd_in, float* pprev,
-                 float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
+int orc_step_f32(const orc_problem* pb, const float* q_in, const float* qd_in, const float* f_in,
+                 float* pprev, float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
                  float* rho_out, float* p_out, int* nbr_count_out, int* iters_out,
                  double* err_out, int nthreads) {
-    return orc_step_impl_f32(pb, q_in, qd_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
+    return orc_step_impl_f32(pb, q_in, qd_in, f_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
                              rho_out, p_out, nbr_count_out, iters_out, err_out, nthreads);
 }
 
-int orc_step_f64(const orc_problem* pb, const float* q_in, const float* qd_in, float* pprev,
-                 float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
+int orc_step_f64(const orc_problem* pb, const float* q_in, const float* qd_in, const float* f_in,
+                 float* pprev, float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
                  float* rho_out, float* p_out, int* nbr_count_out, int* iters_out,
                  double* err_out, int nthreads) {
-    return orc_step_impl_f64(pb, q_in, qd_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
+    return orc_step_impl_f64(pb, q_in, qd_in, f_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
                              rho_out, p_out, nbr_count_out, iters_out, err_out, nthreads);
 }
 
@@ -358,6 +362,15 @@ void orc_kernel_f64(double H, double r2, double* W, double* gf) {
     const float g[3] = {0.f, 0.f, 0.f};
     make_phys_f64(&P, H, 0.0, 1000.0, 0.0, g);
     kern_wg_f64(&P, r2, W, gf);
+}
+
+/* Density and viscous acceleration of one state, particle order (see orc_viscosity_impl). */
+int orc_viscosity_f64(const orc_problem* pb, const float* q_in, const float* qd_in, double* av, double* rho) {
+    return orc_viscosity_impl_f64(pb, q_in, qd_in, av, rho);
+}
+
+int orc_viscosity_f32(const orc_problem* pb, const float* q_in, const float* qd_in, double* av, double* rho) {
+    return orc_viscosity_impl_f32(pb, q_in, qd_in, av, rho);
 }
 
 /* IISPH operator pieces in FP64 (see orc_operator_impl). */

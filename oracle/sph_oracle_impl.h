@@ -84,9 +84,97 @@ static REAL FN(dot3)(REAL ax, REAL ay, REAL az, REAL bx, REAL by, REAL bz) {
     return (ax * bx + ay * by) + az * bz;
 }
 
+/* Density (Neumaier-compensated, list then boundary order), a_ii and the boundary gradient
+ * sum of every sorted fluid particle (the density half of k_sph_nbr_density). */
+static void FN(pass_density)(const FN(phys_t) * P, const orc_layout* Lp, const REAL* xs, REAL* rho, REAL* aii,
+                             REAL* gb, REAL dt2, int nthreads) {
+    const orc_layout L = *Lp;
+    const int Nf = L.nf;
+    const int* nbr = L.nbr;
+    const int* ncnt = L.ncnt;
+    const int cap = L.cap;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int s = 0; s < Nf; ++s) {
+        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
+        const REAL xi = xs[4 * s], yi = xs[4 * s + 1], zi = xs[4 * s + 2], mi = xs[4 * s + 3];
+        REAL sum = mi * Pw->W0, cmp = 0, gfx = 0, gfy = 0, gfz = 0, S = 0;
+        for (int k = 0; k < ncnt[s]; ++k) {
+            const int j = nbr[(size_t)s * cap + k];
+            const REAL dx = xi - xs[4 * j], dy = yi - xs[4 * j + 1], dz = zi - xs[4 * j + 2];
+            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
+            REAL Wv, gf;
+            FN(kern_wg)(Pw, r2, &Wv, &gf);
+            const REAL mj = xs[4 * j + 3];
+            FN(neumaier)(&sum, &cmp, mj * Wv);
+            const REAL gx = gf * dx, gy = gf * dy, gz = gf * dz;
+            gfx += mj * gx;
+            gfy += mj * gy;
+            gfz += mj * gz;
+            S += mj * FN(dot3)(gx, gy, gz, gx, gy, gz);
+        }
+        REAL gbx = 0, gby = 0, gbz = 0;
+        for (int k = L.bstart[s]; k < L.bstart[s + 1]; ++k) {
+            const int b = L.blist[k];
+            const REAL dx = xi - (REAL)L.bxs[3 * b], dy = yi - (REAL)L.bxs[3 * b + 1],
+                       dz = zi - (REAL)L.bxs[3 * b + 2];
+            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
+            REAL Wv, gf;
+            FN(kern_wg)(Pw, r2, &Wv, &gf);
+            FN(neumaier)(&sum, &cmp, Pw->psi * Wv);
+            gbx += gf * dx;
+            gby += gf * dy;
+            gbz += gf * dz;
+        }
+        const REAL r = sum + cmp;
+        const REAL Gx = gfx + Pw->psi * gbx, Gy = gfy + Pw->psi * gby, Gz = gfz + Pw->psi * gbz;
+        const REAL num = FN(dot3)(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
+        rho[s] = r;
+        aii[s] = -(dt2 * (num / (r * r)));
+        gb[3 * s] = gbx;
+        gb[3 * s + 1] = gby;
+        gb[3 * s + 2] = gbz;
+    }
+}
+
+/* Viscosity a_visc = 10 nu sum_j (m_j/rho_j) (v_ij . x_ij) / (r^2 + 0.01 H^2) gradW_ij and the
+ * advected velocity v + dt (g + a_visc) (k_sph_advect). */
+static void FN(pass_viscosity)(const FN(phys_t) * P, const orc_layout* Lp, const REAL* xs, const REAL* vs,
+                               const REAL* rho, REAL* av, REAL* va, REAL rdt, int nthreads) {
+    const orc_layout L = *Lp;
+    const int Nf = L.nf;
+    const int* nbr = L.nbr;
+    const int* ncnt = L.ncnt;
+    const int cap = L.cap;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int s = 0; s < Nf; ++s) {
+        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
+        const REAL xi = xs[4 * s], yi = xs[4 * s + 1], zi = xs[4 * s + 2];
+        const REAL vxi = vs[3 * s], vyi = vs[3 * s + 1], vzi = vs[3 * s + 2];
+        REAL ax = 0, ay = 0, az = 0;
+        for (int k = 0; k < ncnt[s]; ++k) {
+            const int j = nbr[(size_t)s * cap + k];
+            const REAL dx = xi - xs[4 * j], dy = yi - xs[4 * j + 1], dz = zi - xs[4 * j + 2];
+            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
+            const REAL gf = FN(kern_gf)(Pw, r2);
+            const REAL vx = FN(dot3)(vxi - vs[3 * j], vyi - vs[3 * j + 1], vzi - vs[3 * j + 2], dx, dy, dz);
+            const REAL c = (xs[4 * j + 3] / rho[j]) * (vx / (r2 + Pw->eps2));
+            ax += c * (gf * dx);
+            ay += c * (gf * dy);
+            az += c * (gf * dz);
+        }
+        const REAL avx = Pw->visc10 * ax, avy = Pw->visc10 * ay, avz = Pw->visc10 * az;
+        av[3 * s] = avx;
+        av[3 * s + 1] = avy;
+        av[3 * s + 2] = avz;
+        va[3 * s] = vxi + rdt * (Pw->g[0] + avx);
+        va[3 * s + 1] = vyi + rdt * (Pw->g[1] + avy);
+        va[3 * s + 2] = vzi + rdt * (Pw->g[2] + avz);
+    }
+}
+
 /* One step. See orc_step_* in sph_oracle.c for the argument contract. */
 static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const float* qd_in,
-                             float* pprev, float dt, const int* forced_iters, float* q_out,
+                             const float* f_in, float* pprev, float dt, const int* forced_iters, float* q_out,
                              float* qd_out, float* f_out, float* rho_out, float* p_out,
                              int* nbr_count_out, int* iters_out, double* err_out, int nthreads) {
     const int N = pb->n, W = pb->n_worlds;
@@ -135,74 +223,8 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
     const int* ncnt = L.ncnt;
     const int cap = L.cap;
 
-    /* density, a_ii, boundary gradient sum */
-#pragma omp parallel for num_threads(nthreads) schedule(static)
-    for (int s = 0; s < Nf; ++s) {
-        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
-        const REAL xi = xs[4 * s], yi = xs[4 * s + 1], zi = xs[4 * s + 2], mi = xs[4 * s + 3];
-        REAL sum = mi * Pw->W0, cmp = 0, gfx = 0, gfy = 0, gfz = 0, S = 0;
-        for (int k = 0; k < ncnt[s]; ++k) {
-            const int j = nbr[(size_t)s * cap + k];
-            const REAL dx = xi - xs[4 * j], dy = yi - xs[4 * j + 1], dz = zi - xs[4 * j + 2];
-            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
-            REAL Wv, gf;
-            FN(kern_wg)(Pw, r2, &Wv, &gf);
-            const REAL mj = xs[4 * j + 3];
-            FN(neumaier)(&sum, &cmp, mj * Wv);
-            const REAL gx = gf * dx, gy = gf * dy, gz = gf * dz;
-            gfx += mj * gx;
-            gfy += mj * gy;
-            gfz += mj * gz;
-            S += mj * FN(dot3)(gx, gy, gz, gx, gy, gz);
-        }
-        REAL gbx = 0, gby = 0, gbz = 0;
-        for (int k = L.bstart[s]; k < L.bstart[s + 1]; ++k) {
-            const int b = L.blist[k];
-            const REAL dx = xi - (REAL)L.bxs[3 * b], dy = yi - (REAL)L.bxs[3 * b + 1],
-                       dz = zi - (REAL)L.bxs[3 * b + 2];
-            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
-            REAL Wv, gf;
-            FN(kern_wg)(Pw, r2, &Wv, &gf);
-            FN(neumaier)(&sum, &cmp, Pw->psi * Wv);
-            gbx += gf * dx;
-            gby += gf * dy;
-            gbz += gf * dz;
-        }
-        const REAL r = sum + cmp;
-        const REAL Gx = gfx + Pw->psi * gbx, Gy = gfy + Pw->psi * gby, Gz = gfz + Pw->psi * gbz;
-        const REAL num = FN(dot3)(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
-        rho[s] = r;
-        aii[s] = -(dt2 * (num / (r * r)));
-        gb[3 * s] = gbx;
-        gb[3 * s + 1] = gby;
-        gb[3 * s + 2] = gbz;
-    }
-    /* viscosity, advected velocity */
-#pragma omp parallel for num_threads(nthreads) schedule(static)
-    for (int s = 0; s < Nf; ++s) {
-        const FN(phys_t)* Pw = &P[L.fworld_s[s]];
-        const REAL xi = xs[4 * s], yi = xs[4 * s + 1], zi = xs[4 * s + 2];
-        const REAL vxi = vs[3 * s], vyi = vs[3 * s + 1], vzi = vs[3 * s + 2];
-        REAL ax = 0, ay = 0, az = 0;
-        for (int k = 0; k < ncnt[s]; ++k) {
-            const int j = nbr[(size_t)s * cap + k];
-            const REAL dx = xi - xs[4 * j], dy = yi - xs[4 * j + 1], dz = zi - xs[4 * j + 2];
-            const REAL r2 = (dx * dx + dy * dy) + dz * dz;
-            const REAL gf = FN(kern_gf)(Pw, r2);
-            const REAL vx = FN(dot3)(vxi - vs[3 * j], vyi - vs[3 * j + 1], vzi - vs[3 * j + 2], dx, dy, dz);
-            const REAL c = (xs[4 * j + 3] / rho[j]) * (vx / (r2 + Pw->eps2));
-            ax += c * (gf * dx);
-            ay += c * (gf * dy);
-            az += c * (gf * dz);
-        }
-        const REAL avx = Pw->visc10 * ax, avy = Pw->visc10 * ay, avz = Pw->visc10 * az;
-        av[3 * s] = avx;
-        av[3 * s + 1] = avy;
-        av[3 * s + 2] = avz;
-        va[3 * s] = vxi + rdt * (Pw->g[0] + avx);
-        va[3 * s + 1] = vyi + rdt * (Pw->g[1] + avy);
-        va[3 * s + 2] = vzi + rdt * (Pw->g[2] + avz);
-    }
+    FN(pass_density)(P, &L, xs, rho, aii, gb, dt2, nthreads);
+    FN(pass_viscosity)(P, &L, xs, vs, rho, av, va, rdt, nthreads);
     /* source term, warm start */
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int s = 0; s < Nf; ++s) {
@@ -323,9 +345,16 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
         const int k = L.fperm[s];
         const int id = L.fid[k];
         const REAL wi = (REAL)pb->inv_mass[id];
-        REAL v1x = vs[3 * s] + (fx * wi + Pw->g[0]) * rdt;
-        REAL v1y = vs[3 * s + 1] + (fy * wi + Pw->g[1]) * rdt;
-        REAL v1z = vs[3 * s + 2] + (fz * wi + Pw->g[2]) * rdt;
+        /* total force: the caller's force on the particle (state_in.particle_f) plus SPH */
+        REAL tx = fx, ty = fy, tz = fz;
+        if (f_in) {
+            tx = (REAL)f_in[3 * id] + fx;
+            ty = (REAL)f_in[3 * id + 1] + fy;
+            tz = (REAL)f_in[3 * id + 2] + fz;
+        }
+        REAL v1x = vs[3 * s] + (tx * wi + Pw->g[0]) * rdt;
+        REAL v1y = vs[3 * s + 1] + (ty * wi + Pw->g[1]) * rdt;
+        REAL v1z = vs[3 * s + 2] + (tz * wi + Pw->g[2]) * rdt;
         const REAL mag = RSQRT(FN(dot3)(v1x, v1y, v1z, v1x, v1y, v1z));
         const REAL vmax = (REAL)pb->v_max;
         if (mag > vmax) {
@@ -368,7 +397,10 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
         const float wi = pb->inv_mass[i];
         const float stp = (-wi < 0.0f) ? 1.0f : 0.0f;
         float v1[3];
-        for (int a = 0; a < 3; ++a) v1[a] = qd_in[3 * i + a] + (0.0f * wi + pb->gravity[3 * wd + a] * stp) * dt;
+        for (int a = 0; a < 3; ++a) {
+            const float fa = f_in ? f_in[3 * i + a] : 0.0f;
+            v1[a] = qd_in[3 * i + a] + (fa * wi + pb->gravity[3 * wd + a] * stp) * dt;
+        }
         const float mag = sqrtf((v1[0] * v1[0] + v1[1] * v1[1]) + v1[2] * v1[2]);
         if (mag > pb->v_max) {
             const float sc = pb->v_max / mag;
@@ -499,6 +531,48 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
         for (int a = 0; a < 3; ++a) ap_o[3 * k + a] = (double)ap[3 * s + a];
     }
     free(P); free(xs); free(rho); free(gb); free(pr); free(ap); free(va);
+    orc_layout_free(&L);
+    return 0;
+}
+
+/* Density and viscosity of one state (test hook): a_visc per particle (particle order, 3 per
+ * particle, 0 for non-fluid), computed by the same passes as the step. */
+static int FN(orc_viscosity_impl)(const orc_problem* pb, const float* q_in, const float* qd_in, double* av_o,
+                                  double* rho_o) {
+    const int W = pb->n_worlds;
+    orc_layout L;
+    int rc = orc_layout_build(pb, q_in, &L);
+    if (rc) return rc;
+    const int Nf = L.nf;
+    FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
+    for (int w = 0; w < W; ++w)
+        FN(make_phys)(&P[w], pb->support[w], pb->boundary_volume[w], pb->rest_density,
+                      pb->viscosity, pb->gravity + 3 * w);
+    REAL* xs = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)(Nf + 1));
+    REAL* vs = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* rho = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
+    REAL* aii = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
+    REAL* gb = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* av = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* va = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    for (int s = 0; s < Nf; ++s) {
+        const int id = L.fid[L.fperm[s]];
+        for (int a = 0; a < 3; ++a) {
+            xs[4 * s + a] = (REAL)q_in[3 * id + a];
+            vs[3 * s + a] = (REAL)qd_in[3 * id + a];
+        }
+        xs[4 * s + 3] = (REAL)pb->mass[id];
+    }
+    FN(pass_density)(P, &L, xs, rho, aii, gb, (REAL)0, 1);
+    FN(pass_viscosity)(P, &L, xs, vs, rho, av, va, (REAL)0, 1);
+    for (int i = 0; i < 3 * pb->n; ++i) av_o[i] = 0.0;
+    for (int i = 0; i < pb->n; ++i) rho_o[i] = 0.0;
+    for (int s = 0; s < Nf; ++s) {
+        const int id = L.fid[L.fperm[s]];
+        for (int a = 0; a < 3; ++a) av_o[3 * id + a] = (double)av[3 * s + a];
+        rho_o[id] = (double)rho[s];
+    }
+    free(P); free(xs); free(vs); free(rho); free(aii); free(gb); free(av); free(va);
     orc_layout_free(&L);
     return 0;
 }

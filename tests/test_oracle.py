@@ -218,6 +218,82 @@ def test_viscosity_vanishes_for_uniform_velocity():
     assert fv[top].mean() < 0 < fv[bottom].mean()
 
 
+def _viscosity_continuum_factor(eps2: float) -> float:
+    """c = int W'(r) r^5 / (r^2 + eps2 H^2) dr / int W'(r) r^3 dr for the cubic spline (H = 1).
+
+    The viscosity term 2 (d + 2) nu sum_j V_j (v_ij . x_ij) / (r^2 + eta^2) gradW_ij tends, as
+    the particle spacing shrinks, to c nu (lap v + 2 grad div v) (Espanol & Revenga, Phys. Rev. E
+    67, 026705, 2003): for the shear field v = (a z^2, 0, 0) the angular average of
+    n_x^2 n_z^2 over the sphere (1/15) turns the sum into 2 a nu c, and c = 1 when eta = 0
+    (the 2 (d + 2) normalisation).  Computed by quadrature of the closed-form W', independent
+    of the oracle's code."""
+    from scipy.integrate import quad
+
+    sig = 8.0 / np.pi
+
+    def dW(r):
+        return sig * (18.0 * r * r - 12.0 * r) if r <= 0.5 else -6.0 * sig * (1.0 - r) ** 2
+
+    num = quad(lambda r: dW(r) * r ** 5 / (r * r + eps2), 0.0, 1.0, points=[0.5], epsabs=0, epsrel=1e-12)[0]
+    den = quad(lambda r: dW(r) * r ** 3, 0.0, 1.0, points=[0.5], epsabs=0, epsrel=1e-12)[0]
+    return num / den
+
+
+def _lattice_viscosity(hs: float, field: str):
+    """a_visc of the centre particle of a perfect (2n + 1)^3 fluid lattice, support H = hs s,
+    no boundary, the centre's neighbours all with complete neighbourhoods."""
+    s = 0.01
+    n = int(np.ceil(2.0 * hs)) + 1
+    b = ModelBuilder(gravity=0.0)
+    d = 2 * n + 1
+    b.add_particle_grid(pos=(0, 0, 0), rot=(0, 0, 0, 1), vel=(0, 0, 0), dim_x=d, dim_y=d, dim_z=d,
+                        cell_x=s, cell_y=s, cell_z=s, mass=1000.0 * s ** 3, jitter=0.0)
+    nu, a = 0.25, 3.0
+    m, pb = _problem(b, hs * s, viscosity=nu, max_neighbors=4096)
+    q = pb.q0.astype(np.float64)
+    c = q[np.argmin(np.linalg.norm(q - q.mean(axis=0), axis=1))]
+    qd = np.zeros_like(pb.qd0)
+    if field == "shear":  # v = (a z^2, 0, 0): lap v = (2a, 0, 0), div v = 0
+        qd[:, 0] = a * (q[:, 2] - c[2]) ** 2
+    elif field == "stretch":  # v = (a x^2, 0, 0): lap v + 2 grad div v = (6a, 0, 0)
+        qd[:, 0] = a * (q[:, 0] - c[0]) ** 2
+    else:  # v = (a z, 0, 0): linear, every second derivative 0
+        qd[:, 0] = a * (q[:, 2] - c[2])
+    av, rho = O.viscosity(pb, pb.q0, qd)
+    i = int(np.argmin(np.linalg.norm(q - c, axis=1)))
+    return av[i], rho[i], nu, a
+
+
+@pytest.mark.parametrize("hs", [6.0, 8.0])
+def test_viscosity_magnitude_converges_to_the_laplacian(hs):
+    """Pins the magnitude of the viscosity term: on a fine lattice (H = 6 s and 8 s, about
+    900 and 2100 neighbours) a_visc_x of v = (a z^2, 0, 0) is 2 a nu c and that of
+    v = (a x^2, 0, 0) is 6 a nu c within 1 %, c from quadrature.  A wrong 2 (d + 2) factor
+    (8 or 12 instead of 10), a dropped m_j / rho_j, a dropped regulariser (c -> 1, 3.8 % off)
+    or a wrong sign fails the 1 % bound."""
+    c = _viscosity_continuum_factor(0.01)
+    assert 0.95 < c < 0.97
+    av, rho, nu, a = _lattice_viscosity(hs, "shear")
+    assert rho == pytest.approx(1000.0, rel=2e-3)
+    assert av[0] / (2.0 * a * nu) == pytest.approx(c, rel=1e-2)
+    assert abs(av[1]) < 1e-9 * abs(av[0]) and abs(av[2]) < 1e-9 * abs(av[0])
+    av, rho, nu, a = _lattice_viscosity(hs, "stretch")
+    assert av[0] / (6.0 * a * nu) == pytest.approx(c, rel=1e-2)
+
+
+def test_viscosity_of_linear_field_and_method_resolution():
+    """A linear shear field has no viscous acceleration on a symmetric lattice; at the
+    method's own H = 2 s (26 neighbours) the lattice sum gives 0.70 of the continuum value,
+    rising monotonically towards it with H / s (the discretisation error of the method)."""
+    av, _, nu, a = _lattice_viscosity(2.0, "linear")
+    # a single neighbour's term is of order a nu / s; the float32 lattice is symmetric about the
+    # centre only to rounding, which leaves about 1e-6 of that
+    assert np.abs(av).max() < 1e-5 * a * nu / 0.01
+    c = _viscosity_continuum_factor(0.01)
+    r = [_lattice_viscosity(hs, "shear")[0][0] / (2.0 * a * nu) / c for hs in (2.0, 3.0, 4.0)]
+    assert 0.70 < r[0] < 0.75 and r[0] < r[1] < r[2] <= 1.01
+
+
 # ---- integrator --------------------------------------------------------------------------------
 
 
