@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NSPH_ABI_VERSION 1
+#define NSPH_ABI_VERSION 2
 
 enum nsph_status {
     NSPH_OK = 0,
@@ -197,30 +197,54 @@ typedef struct nsph_step_report {
  * non-zero velocity (moving boundaries are not supported). */
 int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* model, const nsph_sph_params* params,
                        nsph_solver** out);
-/* One step: state_in (d_q_in, d_qd_in) -> state_out (d_q_out, d_qd_out, d_f_out).
- * d_f_out receives the SPH force (viscosity + pressure, N) of fluid particles and 0
- * elsewhere; it may be NULL. Inactive particles: q_out = q_in, qd_out untouched
- * (solver.py:27-29). check_every > 0: after min_iters Jacobi iterations the step's end is
- * enqueued guarded by "every world converged" and the host synchronises once; only if a
- * world has not converged does it iterate on, reading the convergence count every
- * check_every iterations, and enqueue the end again (iteration counts are decided per world
- * on the device, so results do not depend on this schedule). check_every = 0: launch
- * max_iters early-exit iterations without any host read; the step is then capture-safe.
- * report may be NULL; with check_every = 0 and no report the step does not synchronise. */
-int nsph_solver_step(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
-                     float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
+/* The particle arrays of one State (newton/_src/sim/state.py:101-108), particle order, device
+ * pointers.  State in: */
+typedef struct nsph_state_in {
+    const float* d_q;   /* particle_q  (N x 3)                                                 */
+    const float* d_qd;  /* particle_qd (N x 3)                                                 */
+    const float* d_f;   /* particle_f  (N x 3) or NULL: forces the caller applied to the
+                           particles, integrated together with the SPH force as
+                           integrate_particles integrates particle_f (solver.py:24-46)        */
+    const float* d_p;   /* sph_pressure (N) or NULL: previous-step pressure, the input of the
+                           "prev" and "mlp" warm starts (NULL: the solver's own copy of its last
+                           step's pressure); entries of non-fluid particles are not read        */
+} nsph_state_in;
+
+/* State out: */
+typedef struct nsph_state_out {
+    float* d_q;   /* particle_q  (N x 3)                                                       */
+    float* d_qd;  /* particle_qd (N x 3)                                                       */
+    float* d_f;   /* NULL, or receives the SPH force (viscosity + pressure, N) of fluid
+                     particles and 0 elsewhere                                                 */
+    float* d_p;   /* NULL, or receives the step's converged pressure (sph_pressure; 0 for
+                     non-fluid particles)                                                      */
+} nsph_state_out;
+
+/* One step state_in -> state_out (SolverBase.step, solver.py:301-316).  Inactive particles:
+ * q_out = q_in, qd_out untouched (solver.py:27-29).  state_in and state_out must not alias.
+ * check_every > 0: after min_iters Jacobi iterations the step's end is enqueued guarded by
+ * "every world converged" and the host synchronises once; only if a world has not converged
+ * does it iterate on, reading the convergence count every check_every iterations, and enqueue
+ * the end again (iteration counts are decided per world on the device, so results do not
+ * depend on this schedule).  check_every = 0: launch max_iters early-exit iterations without
+ * any host read; the step is then capture-safe.  report may be NULL; with check_every = 0 and
+ * no report the step does not synchronise. */
+int nsph_solver_step(nsph_solver* s, const nsph_state_in* in, const nsph_state_out* out, float dt,
                      int check_every, nsph_step_report* report);
 /* Capture one step from state_in to state_out (fixed pointers and dt) into HIP graphs:
  * prologue + iter_slots Jacobi iterations, and the epilogue. Capture runs on a private
  * stream; nothing is allocated on the device during capture or replay. *handle names the
- * graph pair for nsph_solver_graph_step. */
-int nsph_solver_graph_capture(nsph_solver* s, const float* d_q_in, const float* d_qd_in,
-                              float* d_q_out, float* d_qd_out, float* d_f_out, float dt,
+ * graph pair for nsph_solver_graph_step.  The warm-start mode and MLP in force at capture
+ * are part of the graph. */
+int nsph_solver_graph_capture(nsph_solver* s, const nsph_state_in* in, const nsph_state_out* out, float dt,
                               int iter_slots, int* handle);
 /* Replay a captured step on the context stream. When iter_slots < max_iters the host reads
  * the still-iterating world count once after the slots and, if needed, finishes the solve
- * with eager iterations before the epilogue; results equal nsph_solver_step's. */
+ * with eager iterations before the epilogue; results equal nsph_solver_step's.  Fails with
+ * NSPH_ERR_INVALID when the solver's warm-start mode or MLP differs from the capture's. */
 int nsph_solver_graph_step(nsph_solver* s, int handle, nsph_step_report* report);
+/* Free a captured step (its handle is not reused). */
+int nsph_solver_graph_destroy(nsph_solver* s, int handle);
 /* Collect the report of the last step (synchronises the stream). */
 int nsph_solver_report(nsph_solver* s, nsph_step_report* report);
 /* Per-particle fields of the last step in particle order (any pointer may be NULL):
@@ -234,6 +258,9 @@ int nsph_solver_neighbors(nsph_solver* s, int* d_counts, int* d_idx);
 int nsph_solver_set_warm_start(nsph_solver* s, int mode);
 /* Forget the previous-step pressure (warm start input). */
 int nsph_solver_reset_pressure(nsph_solver* s);
+/* Warm-start pressure p0 of the last step in particle order (the pressure the Jacobi
+ * iterations started from: 0, 0.5 p_prev or the MLP's prediction; 0 for non-fluid). */
+int nsph_solver_warm_pressure(nsph_solver* s, float* d_p0);
 int nsph_solver_num_fluid(const nsph_solver* s, int* n_fluid, int* n_boundary);
 /* Workgroup size of the gather kernels (64, 128 or 256 threads; a 256-particle chunk is split
  * over 256 / gather_block workgroups). Scheduling only: results are identical for every value.

@@ -88,6 +88,18 @@ class StepReport(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class StateIn(C.Structure):
+    """nsph_state_in: device pointers of state_in's particle arrays (q, qd, f, sph pressure)."""
+
+    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p)]
+
+
+class StateOut(C.Structure):
+    """nsph_state_out: device pointers of state_out's particle arrays (q, qd, f, sph pressure)."""
+
+    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _f = C.c_float
@@ -110,10 +122,12 @@ SIGNATURES: dict[str, list] = {
     "nsph_grid_query": [_vp, _vp, _i, _f, _i, _vp, _vp],
     "nsph_grid_destroy": [_vp],
     "nsph_solver_create": [_vp, C.POINTER(ModelDesc), C.POINTER(SphParams), C.POINTER(_vp)],
-    "nsph_solver_step": [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, C.POINTER(StepReport)],
+    "nsph_solver_step": [_vp, C.POINTER(StateIn), C.POINTER(StateOut), _f, _i, C.POINTER(StepReport)],
     "nsph_solver_report": [_vp, C.POINTER(StepReport)],
-    "nsph_solver_graph_capture": [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, _pi],
+    "nsph_solver_graph_capture": [_vp, C.POINTER(StateIn), C.POINTER(StateOut), _f, _i, _pi],
     "nsph_solver_graph_step": [_vp, _i, C.POINTER(StepReport)],
+    "nsph_solver_graph_destroy": [_vp, _i],
+    "nsph_solver_warm_pressure": [_vp, _vp],
     "nsph_solver_fields": [_vp, _vp, _vp, _vp],
     "nsph_solver_world_stats": [_vp, _pi, _pd, _pi],
     "nsph_solver_neighbors": [_vp, _vp, _vp],

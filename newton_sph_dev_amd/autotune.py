@@ -58,7 +58,7 @@ def apply(solver, cfg: dict) -> None:
     else:
         solver.use_graph = False
         solver.check_every = cfg["check_every"]
-    solver._graphs.clear()
+    solver.clear_graphs()
 
 
 def _time(solver, a, b, dt, steps) -> tuple[float, object, object]:

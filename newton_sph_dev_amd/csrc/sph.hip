@@ -133,6 +133,7 @@ struct SphDev {
     int* mixed;  // per world: 1 when its fluid particles do not all have one mass
     float4* ap;
     float* pws;  // warm-start input in sorted order
+    float* p0;   // warm-start pressure the iterations started from, sorted order
     int4* nbr4;
     int* ncnt;
     float2* aux;  // (boundary neighbours, distance to the nearest boundary particle / H, 1 if none)
@@ -297,6 +298,7 @@ __device__ inline void mark_cell_wave(const uint32_t* __restrict__ skeys, int s,
 __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restrict__ qd,
                               const float* __restrict__ mass, const int* __restrict__ fid,
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
+                              const float* __restrict__ p_in,
                               const uint32_t* __restrict__ skeys, int n, float4* __restrict__ xs,
                               float4* __restrict__ vs, float* __restrict__ pws,
                               int2* __restrict__ ftable, int* __restrict__ fcend,
@@ -308,7 +310,7 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
         inv[k] = s;
         xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
         vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
-        pws[s] = pprev[k];
+        pws[s] = p_in ? p_in[id] : pprev[k];
     }
     mark_cell_wave(skeys, s, n, ftable, fcend);
 }
@@ -850,6 +852,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
     D.src[s] = (P.rho0 - rho) - dt * sum;
     D.p[s] = p0;
+    D.p0[s] = p0;
     D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pri);
     const float cb = P.psi * pri;
     D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
@@ -878,6 +881,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
     if (warm == NSPH_WARM_PREV) p0 = 0.5f * D.pws[s];
     else if (warm == NSPH_WARM_MLP) p0 = D.pws[s];
     D.p[s] = p0;
+    D.p0[s] = p0;
     D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
 }
 
@@ -1092,8 +1096,9 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const int* __rest
 __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const int* __restrict__ inv,
                                 const int* __restrict__ fid, const int* __restrict__ fworld,
                                 const float* __restrict__ inv_mass, int n, float dt, float v_max,
-                                float* __restrict__ q_out, float* __restrict__ qd_out,
-                                float* __restrict__ f_out, float* __restrict__ pprev) {
+                                const float* __restrict__ f_in, float* __restrict__ q_out,
+                                float* __restrict__ qd_out, float* __restrict__ f_out,
+                                float* __restrict__ p_out, float* __restrict__ pprev) {
     if (guard && *guard != 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -1104,9 +1109,16 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
     const float4 xi = D.xs[s];
     const float4 v0 = D.vs[s];
     const float wi = inv_mass[id];
-    float v1x = v0.x + (f.x * wi + P.g[0]) * dt;
-    float v1y = v0.y + (f.y * wi + P.g[1]) * dt;
-    float v1z = v0.z + (f.z * wi + P.g[2]) * dt;
+    // total force: the caller's (state_in.particle_f) plus the SPH force
+    float tx = f.x, ty = f.y, tz = f.z;
+    if (f_in) {
+        tx = f_in[3 * id] + f.x;
+        ty = f_in[3 * id + 1] + f.y;
+        tz = f_in[3 * id + 2] + f.z;
+    }
+    float v1x = v0.x + (tx * wi + P.g[0]) * dt;
+    float v1y = v0.y + (ty * wi + P.g[1]) * dt;
+    float v1z = v0.z + (tz * wi + P.g[2]) * dt;
     const float mag = sqrtf(dot3(v1x, v1y, v1z, v1x, v1y, v1z));
     if (mag > v_max) {
         const float sc = v_max / mag;
@@ -1127,6 +1139,7 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
         f_out[3 * id + 2] = f.z;
     }
     pprev[k] = f.w;
+    if (p_out) p_out[id] = f.w;
     if (!(isfinite(x1) && isfinite(y1) && isfinite(z1) && isfinite(v1x) && isfinite(v1y) &&
           isfinite(v1z)))
         atomicOr(&D.flags[2], 1);
@@ -1221,17 +1234,20 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
         const float rho = D.rho[s];
         const float4 xi = D.xs[s];
         D.p[s] = p0;
+        D.p0[s] = p0;
         D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
     }
 }
 
-// Non-fluid particles, reference integrator with zero force (solver.py:24-46).
+// Non-fluid particles, reference integrator with the caller's force (zero if none)
+// (solver.py:24-46).
 __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const float* __restrict__ q,
                                   const float* __restrict__ qd, const float* __restrict__ inv_mass,
                                   const int* __restrict__ flags, const int* __restrict__ world,
                                   const PhysWorld* __restrict__ phys, int W, float dt, float v_max,
-                                  float* __restrict__ q_out, float* __restrict__ qd_out,
-                                  float* __restrict__ f_out) {
+                                  const float* __restrict__ f_in, float* __restrict__ q_out,
+                                  float* __restrict__ qd_out, float* __restrict__ f_out,
+                                  float* __restrict__ p_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int id = ids[k];
@@ -1241,6 +1257,7 @@ __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const floa
         f_out[3 * id + 1] = 0.f;
         f_out[3 * id + 2] = 0.f;
     }
+    if (p_out) p_out[id] = 0.f;
     if ((flags[id] & NSPH_PARTICLE_ACTIVE) == 0) {
         q_out[3 * id] = x0;
         q_out[3 * id + 1] = y0;
@@ -1252,9 +1269,11 @@ __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const floa
     const PhysWorld& P = phys[wd];
     const float wi = inv_mass[id];
     const float stp = (-wi < 0.0f) ? 1.0f : 0.0f;
-    float v1x = qd[3 * id] + (0.0f * wi + P.g[0] * stp) * dt;
-    float v1y = qd[3 * id + 1] + (0.0f * wi + P.g[1] * stp) * dt;
-    float v1z = qd[3 * id + 2] + (0.0f * wi + P.g[2] * stp) * dt;
+    const float fx = f_in ? f_in[3 * id] : 0.0f, fy = f_in ? f_in[3 * id + 1] : 0.0f,
+                fz = f_in ? f_in[3 * id + 2] : 0.0f;
+    float v1x = qd[3 * id] + (fx * wi + P.g[0] * stp) * dt;
+    float v1y = qd[3 * id + 1] + (fy * wi + P.g[1] * stp) * dt;
+    float v1z = qd[3 * id + 2] + (fz * wi + P.g[2] * stp) * dt;
     const float mag = sqrtf(dot3(v1x, v1y, v1z, v1x, v1y, v1z));
     if (mag > v_max) {
         const float sc = v_max / mag;
@@ -1408,13 +1427,12 @@ static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "adve
                                             "passthrough", "integrate"};
 
 struct StepGraph {
-    const float* q_in;
-    const float* qd_in;
-    float* q_out;
-    float* qd_out;
-    float* f_out;
+    nsph_state_in in;
+    nsph_state_out out;
     float dt;
     int slots;
+    int warm;             // warm-start mode captured
+    const nsph_mlp* mlp;  // MLP whose weights the captured warm start reads
     hipGraphExec_t pro, epi;
 };
 
@@ -1451,7 +1469,7 @@ struct nsph_solver {
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
-    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, pprev, nbr, ncnt, aux;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, p0, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags, wmax;
     int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
     RepDev* d_rep_host = nullptr;  // device address of the pinned RepDev
@@ -1497,6 +1515,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.mixed = bp<int>(S->mixed);
     D.ap = bp<float4>(S->ap);
     D.pws = bp<float>(S->pws);
+    D.p0 = bp<float>(S->p0);
     D.nbr4 = bp<int4>(S->nbr);
     D.ncnt = bp<int>(S->ncnt);
     D.aux = bp<float2>(S->aux);
@@ -1523,7 +1542,7 @@ static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
                       &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
-                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->pprev,
+                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->p0, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags, &S->wmax};
     for (DevBuf* b : bufs) buf_free(*b);
@@ -1771,6 +1790,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->mixed, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
     NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->p0, nf1 * 4));
     NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
     NSPH_TRY(buf_ensure(S->nbr, nf1 * (size_t)S->cap4 * 16));
     NSPH_TRY(buf_ensure(S->ncnt, nf1 * 4));
@@ -1891,8 +1911,9 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* m, const nsph_sph_p
 // them back to back; a captured step holds prologue + a fixed number of iteration slots in
 // one graph and the epilogue in another.
 
-static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
-                            float dt) {
+static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in& in, float dt) {
+    const float* q_in = in.d_q;
+    const float* qd_in = in.d_qd;
     const int Nf = S->Nf, W = S->W, C = S->C;
     const float dt2 = (float)((double)dt * (double)dt);
     SphDev D = make_dev(S);
@@ -1905,7 +1926,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const float* q_in, c
     NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
                         st, &S->skeys, &S->perm));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
-                       S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), S->skeys,
+                       S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p, S->skeys,
                        Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
@@ -1961,37 +1982,36 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
 // guard: nullptr, or the still-iterating count that must be 0 for the fluid part to run.
 // boundary: also integrate the non-fluid particles (unguarded: they do not depend on the
 // pressure solve, so a step that enqueues its end again skips them the second time).
-static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const float* q_in, const float* qd_in,
-                            float* q_out, float* qd_out, float* f_out, float dt,
-                            const int* guard = nullptr, bool boundary = true) {
+static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const nsph_state_in& in, const nsph_state_out& out,
+                            float dt, const int* guard = nullptr, bool boundary = true) {
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
     if (S->Nf > 0) {
         TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
         TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
                                                guard, bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
-                                               S->d_inv_mass, S->Nf, dt, S->v_max, q_out, qd_out, f_out,
-                                               bp<float>(S->pprev)));
+                                               S->d_inv_mass, S->Nf, dt, S->v_max, in.d_f, out.d_q, out.d_qd,
+                                               out.d_f, out.d_p, bp<float>(S->pprev)));
     }
     if (S->No > 0 && boundary) {
         TIMED(KT_PASSTHROUGH,
               hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
-                                 bp<int>(S->oid), S->No, q_in, qd_in, S->d_inv_mass, S->d_flags,
-                                 S->d_world, bp<PhysWorld>(S->phys), W, dt, S->v_max, q_out,
-                                 qd_out, f_out));
+                                 bp<int>(S->oid), S->No, in.d_q, in.d_qd, S->d_inv_mass, S->d_flags,
+                                 S->d_world, bp<PhysWorld>(S->phys), W, dt, S->v_max, in.d_f, out.d_q,
+                                 out.d_qd, out.d_f, out.d_p));
     }
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;
 }
 
-static int check_step_args(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
-                           float* qd_out, float dt) {
-    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_step: NULL solver");
-    NSPH_CHECK(S->N == 0 || (q_in && qd_in && q_out && qd_out), NSPH_ERR_INVALID,
+static int check_step_args(nsph_solver* S, const nsph_state_in* in, const nsph_state_out* out, float dt) {
+    NSPH_CHECK(S != nullptr && in != nullptr && out != nullptr, NSPH_ERR_INVALID, "nsph_solver_step: NULL argument");
+    NSPH_CHECK(S->N == 0 || (in->d_q && in->d_qd && out->d_q && out->d_qd), NSPH_ERR_INVALID,
                "nsph_solver_step: NULL state array");
     NSPH_CHECK(dt > 0.f && std::isfinite(dt), NSPH_ERR_INVALID, "nsph_solver_step: dt must be positive");
-    NSPH_CHECK(S->N == 0 || (q_in != q_out && qd_in != qd_out), NSPH_ERR_INVALID,
-               "nsph_solver_step: state_in and state_out must not alias");
+    NSPH_CHECK(S->N == 0 || (in->d_q != out->d_q && in->d_qd != out->d_qd &&
+                             (in->d_p == nullptr || in->d_p != out->d_p)),
+               NSPH_ERR_INVALID, "nsph_solver_step: state_in and state_out must not alias");
     NSPH_CHECK(S->warm != NSPH_WARM_MLP || S->mlp_ready, NSPH_ERR_INVALID,
                "nsph_solver_step: the MLP warm start needs weights (nsph_solver_set_mlp)");
     return NSPH_OK;
@@ -1999,19 +2019,20 @@ static int check_step_args(nsph_solver* S, const float* q_in, const float* qd_in
 
 extern "C" {
 
-int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
-                     float* qd_out, float* f_out, float dt, int check_every,
-                     nsph_step_report* report) {
-    NSPH_TRY(check_step_args(S, q_in, qd_in, q_out, qd_out, dt));
+int nsph_solver_step(nsph_solver* S, const nsph_state_in* in_, const nsph_state_out* out_, float dt,
+                     int check_every, nsph_step_report* report) {
+    NSPH_TRY(check_step_args(S, in_, out_, dt));
+    const nsph_state_in in = *in_;
+    const nsph_state_out out = *out_;
     NSPH_CHECK(check_every >= 0, NSPH_ERR_INVALID, "nsph_solver_step: check_every must be >= 0");
     hipStream_t st = S->ctx->stream;
     if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
-    NSPH_TRY(enqueue_prologue(S, st, q_in, qd_in, dt));
+    NSPH_TRY(enqueue_prologue(S, st, in, dt));
     int launched = 0;
     if (check_every == 0 || S->Nf == 0) {
         // no host checks: every iteration slot is enqueued (converged worlds skip them)
         NSPH_TRY(enqueue_iterations(S, st, dt, 0, S->max_iters, 0, &launched));
-        NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt));
+        NSPH_TRY(enqueue_epilogue(S, st, in, out, dt));
         S->last_iters_launched = launched;
         S->stepped = 1;
         if (report) NSPH_TRY(collect_report(S, report));
@@ -2026,8 +2047,7 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
     for (bool boundary = true;; boundary = false) {
         const bool last = launched >= S->max_iters;
         const int slot = (launched - 1) & 1;
-        NSPH_TRY(enqueue_epilogue(S, st, q_in, qd_in, q_out, qd_out, f_out, dt,
-                                  last ? nullptr : bp<int>(S->act) + slot, boundary));
+        NSPH_TRY(enqueue_epilogue(S, st, in, out, dt, last ? nullptr : bp<int>(S->act) + slot, boundary));
         RepDev d;
         NSPH_TRY(summarise_step(S, st, last ? -1 : slot, &d));
         if (last || d.act == 0) {
@@ -2041,9 +2061,9 @@ int nsph_solver_step(nsph_solver* S, const float* q_in, const float* qd_in, floa
     return NSPH_OK;
 }
 
-int nsph_solver_graph_capture(nsph_solver* S, const float* q_in, const float* qd_in, float* q_out,
-                              float* qd_out, float* f_out, float dt, int iter_slots, int* handle) {
-    NSPH_TRY(check_step_args(S, q_in, qd_in, q_out, qd_out, dt));
+int nsph_solver_graph_capture(nsph_solver* S, const nsph_state_in* in, const nsph_state_out* out, float dt,
+                              int iter_slots, int* handle) {
+    NSPH_TRY(check_step_args(S, in, out, dt));
     NSPH_CHECK(handle != nullptr, NSPH_ERR_INVALID, "nsph_solver_graph_capture: NULL handle");
     NSPH_CHECK(iter_slots >= 1 && iter_slots <= S->max_iters, NSPH_ERR_INVALID,
                "nsph_solver_graph_capture: iter_slots must be in [1, max_iters]");
@@ -2052,13 +2072,12 @@ int nsph_solver_graph_capture(nsph_solver* S, const float* q_in, const float* qd
     const int timing = S->timing;
     S->timing = 0;
     StepGraph G{};
-    G.q_in = q_in;
-    G.qd_in = qd_in;
-    G.q_out = q_out;
-    G.qd_out = qd_out;
-    G.f_out = f_out;
+    G.in = *in;
+    G.out = *out;
     G.dt = dt;
     G.slots = iter_slots;
+    G.warm = S->warm;
+    G.mlp = S->warm == NSPH_WARM_MLP ? S->mlp : nullptr;
     hipStream_t cs = S->cap_stream;
     int rc = NSPH_OK;
     for (int part = 0; part < 2 && rc == NSPH_OK; ++part) {
@@ -2070,10 +2089,10 @@ int nsph_solver_graph_capture(nsph_solver* S, const float* q_in, const float* qd
         }
         int launched = 0;
         if (part == 0) {
-            rc = enqueue_prologue(S, cs, q_in, qd_in, dt);
+            rc = enqueue_prologue(S, cs, G.in, dt);
             if (rc == NSPH_OK) rc = enqueue_iterations(S, cs, dt, 0, iter_slots, 0, &launched);
         } else {
-            rc = enqueue_epilogue(S, cs, q_in, qd_in, q_out, qd_out, f_out, dt);
+            rc = enqueue_epilogue(S, cs, G.in, G.out, dt);
         }
         hipError_t e = hipStreamEndCapture(cs, &g);
         if (rc == NSPH_OK && e != hipSuccess) {
@@ -2107,9 +2126,12 @@ int nsph_solver_graph_capture(nsph_solver* S, const float* q_in, const float* qd
 
 int nsph_solver_graph_step(nsph_solver* S, int handle, nsph_step_report* report) {
     NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_graph_step: NULL solver");
-    NSPH_CHECK(handle >= 0 && handle < (int)S->graphs.size(), NSPH_ERR_INVALID,
+    NSPH_CHECK(handle >= 0 && handle < (int)S->graphs.size() && S->graphs[handle].pro, NSPH_ERR_INVALID,
                "nsph_solver_graph_step: unknown graph %d", handle);
     const StepGraph& G = S->graphs[handle];
+    NSPH_CHECK(G.warm == S->warm && G.mlp == (S->warm == NSPH_WARM_MLP ? S->mlp : nullptr), NSPH_ERR_INVALID,
+               "nsph_solver_graph_step: graph %d was captured with another warm start or MLP; capture again",
+               handle);
     hipStream_t st = S->ctx->stream;
     NSPH_HIP(hipGraphLaunch(G.pro, st));
     int launched = G.slots;
@@ -2126,6 +2148,19 @@ int nsph_solver_graph_step(nsph_solver* S, int handle, nsph_step_report* report)
     NSPH_HIP(hipGraphLaunch(G.epi, st));
     S->stepped = 1;
     if (report) NSPH_TRY(collect_report(S, report));
+    return NSPH_OK;
+}
+
+int nsph_solver_graph_destroy(nsph_solver* S, int handle) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_graph_destroy: NULL solver");
+    NSPH_CHECK(handle >= 0 && handle < (int)S->graphs.size(), NSPH_ERR_INVALID,
+               "nsph_solver_graph_destroy: unknown graph %d", handle);
+    StepGraph& G = S->graphs[handle];
+    if (G.pro || G.epi) NSPH_HIP(hipStreamSynchronize(S->ctx->stream));
+    if (G.pro) NSPH_HIP(hipGraphExecDestroy(G.pro));
+    if (G.epi) NSPH_HIP(hipGraphExecDestroy(G.epi));
+    G.pro = nullptr;
+    G.epi = nullptr;
     return NSPH_OK;
 }
 
@@ -2192,6 +2227,21 @@ int nsph_solver_reset_pressure(nsph_solver* S) {
     const size_t n = (size_t)S->Nf + 1;
     hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, S->ctx->stream,
                        bp<float>(S->pprev), n);
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+int nsph_solver_warm_pressure(nsph_solver* S, float* d_p0) {
+    NSPH_CHECK(S && d_p0, NSPH_ERR_INVALID, "nsph_solver_warm_pressure: NULL argument");
+    NSPH_CHECK(S->stepped, NSPH_ERR_NOT_BUILT, "nsph_solver_warm_pressure: no step has run");
+    hipStream_t st = S->ctx->stream;
+    const size_t N = (size_t)S->N;
+    if (N == 0) return NSPH_OK;
+    hipLaunchKernelGGL(k_f32_zero, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, d_p0, N);
+    if (S->Nf > 0)
+        hipLaunchKernelGGL(k_sph_fields, dim3((S->Nf + 255) / 256), dim3(256), 0, st, S->perm,
+                           bp<int>(S->fid), S->Nf, (const float*)nullptr, bp<float>(S->p0), (const int*)nullptr,
+                           (float*)nullptr, d_p0, (int*)nullptr);
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;
 }

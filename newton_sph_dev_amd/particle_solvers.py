@@ -46,12 +46,13 @@ def shape_contact_deltas(model: Model, contacts: Contacts, q: torch.Tensor, qd: 
 class SolverXPBD(SolverBase):
     """Position-based particle solver (particle subset of solver_xpbd.py).
 
-    One step: integrate the particles into state_out, build ``model.particle_grid`` over the
-    predicted positions with radius 2 * particle_max_radius + particle_cohesion, then
-    `iterations` times: zero the deltas, project particle-shape contacts (the contacts of
-    Model.collide) and particle-particle contacts (normal + Coulomb friction on the relative
-    velocity, relaxation `soft_contact_relaxation`) and apply the deltas
-    (v = (x_new - x_init) / dt, clamped).  The position buffers ping-pong between
+    One step: integrate the particles into state_out (with state_in.particle_f, which the
+    solver only reads), build ``model.particle_grid`` over the predicted positions with radius
+    2 * particle_max_radius + particle_cohesion, then `iterations` times: zero the deltas,
+    project particle-shape contacts (the contacts of Model.collide; positional only, as
+    solver_xpbd.py:386-418 -- no penalty force) and particle-particle contacts (normal +
+    Coulomb friction on the relative velocity, relaxation `soft_contact_relaxation`) and
+    apply the deltas (v = (x_new - x_init) / dt, clamped).  The position buffers ping-pong between
     state_out and state_in as the reference does without gradients (solver_xpbd.py:157-201),
     so state_in's particle arrays are overwritten; the final result is in state_out.
     """
@@ -91,15 +92,6 @@ class SolverXPBD(SolverBase):
         particle_q, particle_qd = state_out.particle_q, state_out.particle_qd
         self.particle_q_init = state_in.particle_q.clone()
         particle_deltas = torch.empty_like(state_out.particle_qd)
-        if contacts is not None and model.shape_count > 0 and contacts.soft_contact_max > 0:
-            N.call("nsph_eval_shape_contacts", _ctx(model), n, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
-                   N.ptr(model.particle_radius), N.ptr(model.particle_flags), N.ptr(model.shape_material_ke),
-                   N.ptr(model.shape_material_kd), N.ptr(model.shape_material_kf), N.ptr(model.shape_material_mu),
-                   float(model.soft_contact_ke), float(model.soft_contact_kd), float(model.soft_contact_kf),
-                   float(model.soft_contact_mu), float(model.particle_adhesion), N.ptr(contacts.soft_contact_start),
-                   contacts.soft_contact_max, N.ptr(contacts.soft_contact_shape), N.ptr(contacts.soft_contact_body_pos),
-                   N.ptr(contacts.soft_contact_body_vel), N.ptr(contacts.soft_contact_normal),
-                   N.ptr(state_in.particle_f))
         self.integrate_particles(model, state_in, state_out, dt)
         grid = model.particle_grid
         use_grid = n > 1 and _grid_ready(grid)

@@ -29,6 +29,7 @@ import numpy as np
 import torch
 
 from . import cbor
+from .sim import AttributeNamespace
 
 __all__ = ["RingBuffer", "ViewerFile"]
 
@@ -315,15 +316,32 @@ class ViewerFile:
 
     # -- recording ---------------------------------------------------------------------------
     def record(self, state) -> None:
-        """Snapshot (clone) every array attribute of `state`."""
-        self.history.append({k: v.clone() for k, v in vars(state).items() if isinstance(v, torch.Tensor)})
+        """Snapshot (clone) every array attribute of `state` (viewer_file.py:1123-1133), and
+        the arrays of its custom-attribute namespaces under "namespace:name" keys (the
+        solver state a resume needs, such as the SPH pressure sph:pressure)."""
+        frame = {}
+        for k, v in vars(state).items():
+            if isinstance(v, torch.Tensor):
+                frame[k] = v.clone()
+            elif isinstance(v, AttributeNamespace):
+                for a, t in vars(v).items():
+                    if isinstance(t, torch.Tensor):
+                        frame[f"{k}:{a}"] = t.clone()
+        self.history.append(frame)
 
     def playback(self, state, frame_id: int) -> None:
+        """Restore a snapshot into `state` (viewer_file.py:1135-1149): the arrays `state` has,
+        namespaced ones included."""
         if not 0 <= frame_id < len(self.history):
             print(f"Warning: frame_id {frame_id} is out of bounds. Playback skipped.")
             return
         for name, value in self.history[frame_id].items():
-            if hasattr(state, name):
+            if ":" in name:
+                ns, a = name.split(":", 1)
+                holder = getattr(state, ns, None)
+                if holder is not None and hasattr(holder, a):
+                    setattr(holder, a, value)
+            elif hasattr(state, name):
                 setattr(state, name, value)
 
     def record_model(self, model) -> None:

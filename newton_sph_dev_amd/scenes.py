@@ -43,6 +43,12 @@ DAM_BREAK_64K = DamBreak("dam_break_64k", 0.0125, (32, 32, 64), (1.0, 1.0, 1.0))
 DAM_BREAK_1M = DamBreak("dam_break_1m", 0.004, (64, 128, 128), (2.0, 2.0, 1.0))
 
 
+def _register_sph(b: ModelBuilder) -> None:
+    from .solvers import SolverSPH
+
+    SolverSPH.register_custom_attributes(b)
+
+
 def small_dam_break(nx: int = 12, ny: int = 10, nz: int = 14, spacing: float = 0.0125,
                     box: tuple[float, float, float] = (0.3, 0.3, 0.3)) -> DamBreak:
     return DamBreak(f"dam_break_{nx}x{ny}x{nz}", spacing, (nx, ny, nz), box)
@@ -74,8 +80,10 @@ def box_boundary_points(spacing: float, box: tuple[float, float, float], layers:
 
 def build_dam_break(cfg: DamBreak, builder: ModelBuilder | None = None,
                     rest_density: float = REST_DENSITY) -> ModelBuilder:
-    """Add the fluid lattice first (generator seed 42 + 0), then the boundary."""
+    """Add the fluid lattice first (generator seed 42 + 0), then the boundary.  The SPH
+    solver's State attribute (sph:pressure) is registered."""
     b = builder if builder is not None else ModelBuilder()
+    _register_sph(b)
     s = cfg.spacing
     nx, ny, nz = cfg.fluid_dims
     b.add_particle_grid(pos=(0.5 * s, 0.5 * s, 0.5 * s), rot=(0.0, 0.0, 0.0, 1.0), vel=(0.0, 0.0, 0.0),
@@ -118,6 +126,7 @@ def build_batched(cfg: BatchedWorlds, rest_density: float = REST_DENSITY):
     rng = np.random.default_rng(cfg.seed)
     lo, hi = cfg.count_range
     b = ModelBuilder()
+    _register_sph(b)
     supports = []
     for w in range(cfg.n_worlds):
         if cfg.log_uniform:
@@ -169,6 +178,7 @@ def extract_world(model, w: int) -> ModelBuilder:
     g = model.gravity.detach().cpu().numpy().reshape(-1, 3)[w]
     sel = world == w
     b = ModelBuilder()
+    _register_sph(b)
     b.begin_world(gravity=tuple(float(x) for x in g))
     b.add_particles(q[sel].astype(np.float64), qd[sel].astype(np.float64), mass[sel].astype(np.float64),
                     radius[sel].astype(np.float64), flags[sel])

@@ -108,7 +108,7 @@ class Axis(enum.IntEnum):
 
 
 def quat_to_matrix(q: Sequence[float]) -> np.ndarray:
-    """Rotation matrix of a unit quaternion given as (x, y, z, w), in float64."""
+    """Rotation matrix of a unit quaternion given as (x, y, z, w), in float64 (shape frames)."""
     x, y, z, w = (float(c) for c in q)
     return np.array(
         [
@@ -120,13 +120,88 @@ def quat_to_matrix(q: Sequence[float]) -> np.ndarray:
     )
 
 
+def quat_to_matrix_f32(q: Sequence[float]) -> np.ndarray:
+    """Rotation matrix of a unit quaternion (x, y, z, w) in float32, as the reference's
+    add_particle_grid takes it from wp.quat_to_matrix (builder.py:8090): column k is the
+    rotation of the k-th unit vector by v c + u d + (u x v)(2 w), with c = 2 w^2 - 1,
+    d = 2 (u . v), u = (x, y, z), every operation rounded to float32, no fused multiply-add.
+    Whether the reference's host build contracts these products is not known here, so a
+    non-identity rotation may differ from it in the last float32 bit of an entry."""
+    f = np.float32
+    x, y, z, w = (f(c) for c in q)
+    two = f(2.0)
+    c = two * w * w - f(1.0)
+    w2 = w * two
+    cols = []
+    for v in ((f(1), f(0), f(0)), (f(0), f(1), f(0)), (f(0), f(0), f(1))):
+        d = two * ((x * v[0] + y * v[1]) + z * v[2])
+        cr = (y * v[2] - z * v[1], z * v[0] - x * v[2], x * v[1] - y * v[0])
+        cols.append([(v[k] * c + (x, y, z)[k] * d) + cr[k] * w2 for k in range(3)])
+    return np.array(cols, dtype=np.float32).T
+
+
 def _identity_quat(q: Sequence[float]) -> bool:
     return tuple(float(c) for c in q) == (0.0, 0.0, 0.0, 1.0)
+
+
+class AttributeAssignment(enum.IntEnum):
+    """Which object holds a custom attribute (model.py:47-62)."""
+
+    MODEL = 0
+    STATE = 1
+    CONTROL = 2
+    CONTACT = 3
+
+
+class AttributeFrequency(enum.IntEnum):
+    """Entity count a custom attribute follows (model.py:64-100; the particle-relevant subset
+    keeps the reference's values)."""
+
+    ONCE = 0
+    PARTICLE = 9
+    WORLD = 15
+
+
+class AttributeNamespace:
+    """Container of namespaced custom attributes (model.py:106-125)."""
+
+    def __init__(self, name: str):
+        self._name = name
+
+    def __repr__(self):
+        return f"AttributeNamespace('{self._name}', attributes={[k for k in self.__dict__ if not k.startswith('_')]})"
+
+
+def _attach(obj, name: str, namespace: str | None, value) -> None:
+    if namespace:
+        if not hasattr(obj, namespace):
+            setattr(obj, namespace, AttributeNamespace(namespace))
+        setattr(getattr(obj, namespace), name, value)
+    else:
+        setattr(obj, name, value)
 
 
 class ModelBuilder:
     """Builds a particle Model (particle methods of newton/_src/sim/builder.py) with static
     collision shapes (body = -1) for particle-shape contacts."""
+
+    @dataclass
+    class CustomAttribute:
+        """A per-entity attribute added to the Model at finalize (builder.py:522-570): values
+        default to `default` (scalar or fixed-size vector) and are set per particle through the
+        `custom_attributes` argument of add_particle / add_particles / add_particle_grid
+        (dict of key -> value or per-particle values).  `dtype` is a torch dtype."""
+
+        name: str
+        frequency: AttributeFrequency = AttributeFrequency.PARTICLE
+        dtype: Any = torch.float32
+        default: Any = 0
+        assignment: AttributeAssignment = AttributeAssignment.MODEL
+        namespace: str | None = None
+
+        @property
+        def key(self) -> str:
+            return f"{self.namespace}:{self.name}" if self.namespace else self.name
 
     @dataclass
     class ShapeConfig:
@@ -189,6 +264,32 @@ class ModelBuilder:
         self.default_site_cfg = ModelBuilder.ShapeConfig()
         self.default_site_cfg.mark_as_site()
         self._shapes: list[dict] = []
+        self.custom_attributes: dict[str, ModelBuilder.CustomAttribute] = {}
+        self._custom_values: dict[str, list[tuple[int, Any]]] = {}
+
+    # ---- custom attributes (builder.py:1266-1340) ----------------------------------------
+    def add_custom_attribute(self, attribute: "ModelBuilder.CustomAttribute") -> None:
+        """Define a custom attribute; registering the same key again with the same frequency,
+        dtype and assignment is a no-op, with a different one a ValueError."""
+        key = attribute.key
+        old = self.custom_attributes.get(key)
+        if old is not None:
+            if (old.frequency, old.dtype, old.assignment) != (attribute.frequency, attribute.dtype,
+                                                               attribute.assignment):
+                raise ValueError(f"Custom attribute '{key}' already exists with an incompatible specification")
+            return
+        if attribute.frequency not in (AttributeFrequency.PARTICLE, AttributeFrequency.WORLD, AttributeFrequency.ONCE):
+            raise ValueError(f"Custom attribute '{key}': unsupported frequency {attribute.frequency!r}")
+        self.custom_attributes[key] = attribute
+        self._custom_values[key] = []
+
+    def _record_custom(self, custom_attributes: dict | None, first: int, n: int) -> None:
+        for key, value in (custom_attributes or {}).items():
+            if key not in self.custom_attributes:
+                raise AttributeError(f"Custom attribute '{key}' is not defined (add_custom_attribute)")
+            if self.custom_attributes[key].frequency != AttributeFrequency.PARTICLE:
+                raise ValueError(f"Custom attribute '{key}' is not per particle")
+            self._custom_values[key].append((first, n, value))
 
     # ---- worlds (builder.py:2815-2907) -------------------------------------------------
     @property
@@ -245,6 +346,7 @@ class ModelBuilder:
             radius = self.default_particle_radius
         self._append(np.array([pos], dtype=np.float64), np.array([vel], dtype=np.float64),
                      [mass], [radius], [int(flags)])
+        self._record_custom(custom_attributes, self._count - 1, 1)
         return self._count - 1
 
     def add_particles(self, pos, vel, mass, radius=None, flags=None, custom_attributes: dict | None = None):
@@ -258,6 +360,7 @@ class ModelBuilder:
             flags = np.full(n, int(ParticleFlags.ACTIVE))
         self._append(pos, vel, mass, np.asarray(radius, dtype=np.float64).reshape(n),
                      np.asarray(flags, dtype=np.int64).reshape(n))
+        self._record_custom(custom_attributes, self._count - n, n)
 
     def add_particle_grid(self, pos, rot, vel, dim_x: int, dim_y: int, dim_z: int,
                           cell_x: float, cell_y: float, cell_z: float, mass: float, jitter: float,
@@ -267,8 +370,9 @@ class ModelBuilder:
         """Regular lattice of particles (builder.py:8037-8137).
 
         Lattice order is numpy meshgrid's default 'xy' indexing: y slowest, then x, then z
-        fastest (builder.py:8084-8087). Rotation and translation in float64
-        (builder.py:8090-8091); jitter (rng.random - 0.5) * jitter from
+        fastest (builder.py:8084-8087). The rotation matrix is float32 (wp.quat_to_matrix,
+        builder.py:8090), applied to the float64 points with the translation in float64
+        (builder.py:8091); jitter (rng.random - 0.5) * jitter from
         numpy.random.default_rng(42 + particle_count) (builder.py:8095-8096); radii
         radius_mean (+ standard_normal * radius_std) from the same generator (8098-8103).
         """
@@ -276,7 +380,7 @@ class ModelBuilder:
         py = np.arange(dim_y) * cell_y
         pz = np.arange(dim_z) * cell_z
         points = np.stack(np.meshgrid(px, py, pz)).reshape(3, -1).T
-        rot_mat = np.eye(3) if _identity_quat(rot) else quat_to_matrix(rot)
+        rot_mat = quat_to_matrix_f32(rot)
         points = points @ rot_mat.reshape(3, 3).T + np.array(pos, dtype=np.float64)
         velocity = np.broadcast_to(np.array(vel, dtype=np.float64).reshape(1, 3), points.shape)
         rng = np.random.default_rng(42 + self._count)
@@ -291,7 +395,7 @@ class ModelBuilder:
         fl = np.full(n, int(ParticleFlags.ACTIVE) if flags is None else 0, dtype=np.int64)
         if flags is not None:
             fl[:] = int(flags) if np.isscalar(flags) else np.asarray(flags)
-        self.add_particles(points, velocity, masses, radii, fl)
+        self.add_particles(points, velocity, masses, radii, fl, custom_attributes=custom_attributes)
 
     # ---- finalize (builder.py:9745, 9815-9855, 10655-10668) -----------------------------
     # ---- static shapes (builder.py:5449-5990) ----------------------------------------------
@@ -413,6 +517,14 @@ class ModelBuilder:
             m.shape_material_mu = col("mu", torch.float32)
             m.shape_material_restitution = col("restitution", torch.float32)
             m.shape_label = [sh["label"] for sh in self._shapes]
+        for key, attr in self.custom_attributes.items():
+            count = {AttributeFrequency.PARTICLE: n, AttributeFrequency.WORLD: W,
+                     AttributeFrequency.ONCE: 1}[attr.frequency]
+            dflt = torch.as_tensor(attr.default, dtype=attr.dtype)
+            t = dflt.expand((count,) + tuple(dflt.shape)).clone()
+            for first, cnt, value in self._custom_values[key]:
+                t[first:first + cnt] = torch.as_tensor(np.asarray(value), dtype=attr.dtype)
+            m.add_attribute(attr.name, t.to(device), attr.frequency, attr.assignment, attr.namespace)
         if n > 1 and m.particle_max_radius > 0.0:
             from .hashgrid import HashGrid
 
@@ -445,8 +557,14 @@ def _world_starts(world: np.ndarray, world_count: int) -> np.ndarray:
 class Model:
     """Particle part of newton.Model (newton/_src/sim/model.py:140-187)."""
 
+    AttributeAssignment = AttributeAssignment
+    AttributeFrequency = AttributeFrequency
+    AttributeNamespace = AttributeNamespace
+
     def __init__(self, device: torch.device | str = "cpu"):
         self.device = torch.device(device)
+        self.attribute_frequency: dict[str, AttributeFrequency] = {}
+        self.attribute_assignment: dict[str, AttributeAssignment] = {}
         self.requires_grad = False
         self.world_count = 0
         self.particle_q = None
@@ -493,12 +611,35 @@ class Model:
     def particle_count(self) -> int:
         return 0 if self.particle_q is None else int(self.particle_q.shape[0])
 
+    def add_attribute(self, name: str, attrib, frequency, assignment=None, namespace: str | None = None) -> None:
+        """Add a custom attribute (model.py:1112-1160); STATE ones are cloned into every
+        state(), as model.py:875 does."""
+        if isinstance(attrib, torch.Tensor) and attrib.device != self.device:
+            raise AttributeError(f"Attribute '{name}' device mismatch (model={self.device}, got={attrib.device})")
+        holder = getattr(self, namespace, None) if namespace else self
+        if holder is not None and hasattr(holder, name):
+            raise AttributeError(f"Attribute already exists: {namespace + '.' if namespace else ''}{name}")
+        _attach(self, name, namespace, attrib)
+        key = f"{namespace}:{name}" if namespace else name
+        self.attribute_frequency[key] = frequency
+        if assignment is not None:
+            self.attribute_assignment[key] = assignment
+
+    def _add_custom_attributes(self, destination, assignment: AttributeAssignment) -> None:
+        for key in self.attribute_frequency:
+            if self.attribute_assignment.get(key, AttributeAssignment.MODEL) != assignment:
+                continue
+            namespace, name = key.split(":", 1) if ":" in key else (None, key)
+            src = getattr(getattr(self, namespace), name) if namespace else getattr(self, name)
+            _attach(destination, name, namespace, src.clone() if isinstance(src, torch.Tensor) else src)
+
     def state(self, requires_grad: bool | None = None) -> State:
         s = State()
         if self.particle_count:
             s.particle_q = self.particle_q.clone()
             s.particle_qd = self.particle_qd.clone()
             s.particle_f = torch.zeros_like(self.particle_qd)
+        self._add_custom_attributes(s, AttributeAssignment.STATE)
         return s
 
     def control(self, requires_grad: bool | None = None, clone_variables: bool = True) -> Control:
@@ -572,12 +713,31 @@ class State:
             self.particle_f.zero_()
 
     def assign(self, other: State) -> None:
+        """Copy the arrays of another state into this one (state.py:169-229), custom
+        namespaced attributes included."""
         for name in ("particle_q", "particle_qd", "particle_f"):
             dst, src = getattr(self, name), getattr(other, name)
             if (dst is None) != (src is None):
                 raise ValueError(f"State array '{name}' is present in only one of the two states.")
             if dst is not None:
                 dst.copy_(src)
+        ns_self = {k: v for k, v in self.__dict__.items() if isinstance(v, AttributeNamespace)}
+        ns_other = {k: v for k, v in other.__dict__.items() if isinstance(v, AttributeNamespace)}
+        for ns in ns_self.keys() | ns_other.keys():
+            a, b = ns_self.get(ns), ns_other.get(ns)
+            if a is None or b is None:
+                held = (b if a is None else a).__dict__.values()
+                if any(isinstance(v, torch.Tensor) for v in held):
+                    raise ValueError(f"State namespace '{ns}' holds arrays in only one of the two states.")
+                continue
+            for name in a.__dict__.keys() | b.__dict__.keys():
+                if name.startswith("_"):
+                    continue
+                dst, src = a.__dict__.get(name), b.__dict__.get(name)
+                if isinstance(dst, torch.Tensor) != isinstance(src, torch.Tensor):
+                    raise ValueError(f"State array '{ns}.{name}' is present in only one of the two states.")
+                if isinstance(dst, torch.Tensor):
+                    dst.copy_(src)
 
 
 class Control:

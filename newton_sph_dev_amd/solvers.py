@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _native as N
-from .sim import Contacts, Control, Model, ModelBuilder, State
+from .sim import AttributeAssignment, AttributeFrequency, Contacts, Control, Model, ModelBuilder, State
 
 
 class SolverNotifyFlags(enum.IntFlag):
@@ -171,6 +171,16 @@ class SolverSPH(SolverBase):
     Walls are boundary particles.  Static collision shapes (ModelBuilder.add_shape_*) act
     through `contacts` (from Model.collide): after the SPH step each fluid particle in contact
     is projected out of the shapes with the XPBD particle-shape correction.
+
+    State: state_in.particle_f (forces the caller applied) is integrated together with the SPH
+    force, as integrate_particles integrates particle_f (solver.py:24-46); the step writes
+    neither particle_f array (clear_forces() between steps, as the reference's loops do), and
+    the SPH force of the last step is `sph_force` (N x 3, particle order).  With the custom
+    attribute of register_custom_attributes, each State carries the step's pressure in
+    ``state.sph.pressure`` (particle order): the step reads the warm-start input from
+    state_in and writes the converged pressure to state_out, so a recorded State resumes bit
+    for bit whatever the warm start.  Without it the solver keeps its own copy of the last
+    step's pressure.
     """
 
     def __init__(self, model: Model, smoothing_length: float | Sequence[float], *,
@@ -249,10 +259,22 @@ class SolverSPH(SolverBase):
         N.call("nsph_solver_create", self._ctx, C.byref(md), C.byref(pr), C.byref(h))
         self._handle = h
         N.call("nsph_solver_set_launch", h, self.gather_block)
+        # SPH force (viscosity + pressure) of the last step, particle order; the reference's
+        # solvers leave state_out.particle_f to the caller, so the step does too
+        self.sph_force = torch.zeros((m.particle_count, 3), dtype=torch.float32, device=m.device)
         nf, nb = C.c_int(), C.c_int()
         N.call("nsph_solver_num_fluid", h, C.byref(nf), C.byref(nb))
         self.fluid_count = nf.value
         self.boundary_count = nb.value
+
+    @classmethod
+    def register_custom_attributes(cls, builder: ModelBuilder) -> None:
+        """Per-particle State attribute ``sph:pressure`` (solver.py:358-366 hook; custom
+        attributes as builder.py:1266): the pressure of the last step, the warm-start input
+        of the next, carried in the State so that recordings and resumes keep it."""
+        builder.add_custom_attribute(ModelBuilder.CustomAttribute(
+            name="pressure", frequency=AttributeFrequency.PARTICLE, dtype=torch.float32, default=0.0,
+            assignment=AttributeAssignment.STATE, namespace="sph"))
 
     def set_mlp(self, mlp) -> None:
         """Attach a learned warm start (newton_sph_dev_amd.learned.WarmStartMLP) and switch
@@ -268,27 +290,56 @@ class SolverSPH(SolverBase):
             raise ValueError("warm_start='mlp' needs set_mlp() first")
         N.call("nsph_solver_set_warm_start", self._handle, _WARM[mode])
         self.warm_start = mode
+        self.clear_graphs()
+
+    def clear_graphs(self) -> None:
+        """Free every captured step (they hold the warm start, MLP and launch configuration
+        of their capture)."""
+        for h in self._graphs.values():
+            if self._handle is not None:
+                N.call("nsph_solver_graph_destroy", self._handle, int(h))
+        self._graphs.clear()
+        self._graph_refs = []
 
     def notify_model_changed(self, flags: int) -> None:
-        """MODEL_PROPERTIES: re-create (re-classify particles, re-hash the static boundary)."""
+        """MODEL_PROPERTIES: re-create (re-classify particles, re-hash the static boundary);
+        captured steps are dropped with the old solver."""
         if flags & SolverNotifyFlags.MODEL_PROPERTIES:
+            self._graphs.clear()
+            self._graph_refs = []
             self._destroy()
             self._create()
 
     # ---- step ---------------------------------------------------------------------------------
+    @staticmethod
+    def _pressure_of(state: State):
+        ns = getattr(state, "sph", None)
+        return getattr(ns, "pressure", None) if ns is not None else None
+
+    def _io(self, state_in: State, state_out: State) -> tuple:
+        m = self.model
+        for t in (state_in.particle_q, state_in.particle_qd, state_out.particle_q, state_out.particle_qd):
+            if t is None or t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous():
+                raise ValueError("state arrays must be contiguous float32 tensors on the model device")
+        p_in, p_out = self._pressure_of(state_in), self._pressure_of(state_out)
+        for t in (state_in.particle_f, state_out.particle_f, p_in, p_out):
+            if t is not None and (t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous()):
+                raise ValueError("particle_f / sph.pressure must be contiguous float32 tensors on the model device")
+        si = N.StateIn(N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd), N.ptr(state_in.particle_f),
+                       N.ptr(p_in))
+        so = N.StateOut(N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(self.sph_force),
+                        N.ptr(p_out))
+        return si, so
+
     def step(self, state_in: State, state_out: State, control: Control | None,
              contacts: Contacts | None, dt: float) -> None:
         m = self.model
         if m.particle_count == 0:
             return
         N.set_stream(self._ctx, torch.cuda.current_stream(m.device).cuda_stream)
-        for t in (state_in.particle_q, state_in.particle_qd, state_out.particle_q, state_out.particle_qd):
-            if t is None or t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous():
-                raise ValueError("state arrays must be contiguous float32 tensors on the model device")
+        si, so = self._io(state_in, state_out)
         if self.use_graph:
-            key = (state_in.particle_q.data_ptr(), state_in.particle_qd.data_ptr(),
-                   state_out.particle_q.data_ptr(), state_out.particle_qd.data_ptr(),
-                   None if state_out.particle_f is None else state_out.particle_f.data_ptr(), float(dt))
+            key = (si.d_q, si.d_qd, si.d_f, si.d_p, so.d_q, so.d_qd, so.d_f, so.d_p, float(dt))
             h = self._graphs.get(key)
             if h is None:
                 h = self.capture(state_in, state_out, dt, self.graph_slots)
@@ -296,9 +347,8 @@ class SolverSPH(SolverBase):
             self.replay(h)
         else:
             rep = N.StepReport() if self.report else None
-            N.call("nsph_solver_step", self._handle, N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd),
-                   N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(state_out.particle_f),
-                   float(dt), self.check_every, C.byref(rep) if rep is not None else None)
+            N.call("nsph_solver_step", self._handle, C.byref(si), C.byref(so), float(dt), self.check_every,
+                   C.byref(rep) if rep is not None else None)
             self._check_report(rep)
         if contacts is not None and m.shape_count > 0 and getattr(contacts, "soft_contact_max", 0) > 0:
             self._shape_projection(state_out, contacts, dt)
@@ -336,10 +386,9 @@ class SolverSPH(SolverBase):
         and keep their storage; dt is fixed at capture."""
         N.set_stream(self._ctx, torch.cuda.current_stream(self.model.device).cuda_stream)
         slots = self.max_iters if iter_slots is None else int(iter_slots)
+        si, so = self._io(state_in, state_out)
         h = C.c_int()
-        N.call("nsph_solver_graph_capture", self._handle, N.ptr(state_in.particle_q),
-               N.ptr(state_in.particle_qd), N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd),
-               N.ptr(state_out.particle_f), float(dt), slots, C.byref(h))
+        N.call("nsph_solver_graph_capture", self._handle, C.byref(si), C.byref(so), float(dt), slots, C.byref(h))
         self._graph_refs = getattr(self, "_graph_refs", []) + [(state_in, state_out)]
         return h.value
 
@@ -354,7 +403,7 @@ class SolverSPH(SolverBase):
         """Gather-kernel workgroup size (64/128/256); results do not depend on it."""
         N.call("nsph_solver_set_launch", self._handle, int(gather_block))
         self.gather_block = int(gather_block)
-        self._graphs.clear()
+        self.clear_graphs()
 
     def collect_report(self) -> dict:
         rep = N.StepReport()
@@ -392,7 +441,15 @@ class SolverSPH(SolverBase):
         return {"iterations": it, "density_error": err, "state": st}
 
     def reset_pressure(self) -> None:
+        """Forget the solver's own copy of the last pressure (States carrying sph.pressure
+        keep theirs)."""
         N.call("nsph_solver_reset_pressure", self._handle)
+
+    def warm_pressure(self) -> torch.Tensor:
+        """Pressure the last step's Jacobi iterations started from (particle order)."""
+        p0 = torch.empty(self.model.particle_count, dtype=torch.float32, device=self.model.device)
+        N.call("nsph_solver_warm_pressure", self._handle, N.ptr(p0))
+        return p0
 
     def _destroy(self):
         h = getattr(self, "_handle", None)

@@ -10,6 +10,7 @@ differs from the GPU's cell order, hence tolerance (not bit) parity.
   particle_force           newton/_src/solvers/semi_implicit/kernels_contact.py:11-32
   eval_particle_contact    newton/_src/solvers/semi_implicit/kernels_contact.py:35-84
   xpbd_step                particle path of newton/_src/solvers/xpbd/solver_xpbd.py:258-441, 504-505, 657-660
+                           (shape contacts: positional projection only, solver_xpbd.py:386-418)
   semi_implicit_step       particle path of newton/_src/solvers/semi_implicit/solver_semi_implicit.py:120-177
 
 Pinned in tests/test_contacts_oracle.py by closed forms (two-particle normal and friction
@@ -132,9 +133,16 @@ def eval_particle_contact(x, v, radius, flags, ke, kd, kf, mu, cohesion, f_in):
 
 
 def xpbd_step(x, v, f, inv_mass, radius, flags, world, gravity, dt, v_max, mu, cohesion,
-              iterations, relaxation=0.9):
+              iterations, relaxation=0.9, shape_contacts=None, shape_mu=None, soft_contact_mu=0.5,
+              particle_adhesion=0.0):
     """Particle path of one SolverXPBD.step: integrate, then `iterations` rounds of
-    contacts + apply_deltas against the pre-step positions.  Returns (x, v)."""
+    particle-shape corrections (shape_contacts: the (particle, shape, body_pos, body_vel,
+    normal) list of Model.collide, fixed during the step, projected against the current
+    positions as solve_particle_shape_contacts does, xpbd/kernels.py:114-222, launched at
+    solver_xpbd.py:386-418) plus particle-particle contacts, then apply_deltas against the
+    pre-step positions.  f is only read (integrate_particles).  Returns (x, v)."""
+    from . import shapes_oracle as S
+
     x0 = np.asarray(x, np.float64)
     xp, vp = integrate_particles(x, v, f, inv_mass, flags, world, gravity, dt, v_max)
     # the reference's ping-pong buffers: first target is state_in's (x0, v0)
@@ -143,6 +151,9 @@ def xpbd_step(x, v, f, inv_mass, radius, flags, world, gravity, dt, v_max, mu, c
     q, qd = xp, vp
     for _ in range(iterations):
         d = xpbd_particle_contacts(q, qd, inv_mass, radius, flags, mu, cohesion, dt, relaxation)
+        if shape_contacts:
+            d = d + S.xpbd_shape_deltas(q, qd, inv_mass, radius, flags, shape_contacts, shape_mu,
+                                        soft_contact_mu, particle_adhesion, dt, relaxation)
         tgt = 1 - cur
         q, qd = apply_particle_deltas(x0, q, flags, d, dt, v_max, bufs[tgt][0], bufs[tgt][1])
         bufs[tgt] = (q, qd)

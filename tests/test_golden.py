@@ -92,7 +92,7 @@ def test_gpu_step_against_the_fixture():
     f = s.fields()
     rho = f["density"].cpu().numpy()[fid].astype(np.float64)
     assert np.max(np.abs(rho - fx["rho"]) / fx["rho"]) <= 1e-5
-    force = b.particle_f.cpu().numpy()[fid].astype(np.float64)
+    force = s.sph_force.cpu().numpy()[fid].astype(np.float64)
     assert np.abs(force - fx["f"]).max() / np.abs(fx["f"]).max() <= 1e-5
     counts, idx = (t.cpu().numpy() for t in s.neighbors())
     got = np.concatenate([np.sort(idx[: counts[i], i]) for i in fid])

@@ -64,7 +64,7 @@ def test_batched_worlds_equal_fp32_restatement():
     pb = O.problem_from_model(m, s)
     r = O.step(pb, pb.q0, pb.qd0, precision="f32")
     assert np.array_equal(r["q"], c.particle_q.cpu().numpy())
-    assert np.array_equal(r["f"], c.particle_f.cpu().numpy())
+    assert np.array_equal(r["f"], s.sph_force.cpu().numpy())
     assert np.array_equal(r["iters"], s.world_stats()["iterations"])
 
 

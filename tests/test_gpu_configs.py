@@ -1,0 +1,203 @@
+"""The benchmark configurations on the GPU, at their full sizes, against the oracle.
+
+  C2  dam break, 1,048,576 fluid particles (the bench workload): one step from the initial
+      state equals the same-order FP32 restatement bit for bit in x, v, f, rho, p and the
+      iteration count; density and force are within 1e-5 of the FP64 oracle run with the same
+      iteration count; neighbour lists equal the oracle's, slot for slot.
+  C3  4096 worlds (256..8192 fluid particles, log-uniform), one step: every world converges;
+      16 seeded sample worlds each equal the same world run alone, and the FP32 restatement,
+      bit for bit.
+  C4  the shipped network as warm start on the 1M dam break: 12 steps, each converged to the
+      0.1 % mean compression; the step after the warm start equals the FP32 restatement
+      started from the same warm-start pressure (the network's prediction).
+  C5  1024 worlds (4096..65536 fluid particles, uniform) with the learned warm start in captured
+      (HIP graph) steps: three replays equal three eager steps bit for bit; sampled worlds
+      equal the FP32 restatement from the same warm-start pressure, bit for bit.
+"""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from newton_sph_dev_amd import SolverSPH, scenes
+from newton_sph_dev_amd.learned import WarmStartMLP
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+MLP_PATH = Path(__file__).resolve().parents[1] / "newton_sph_dev_amd" / "data" / "warmstart_mlp.npy"
+REL = 1e-5
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _fluid_order(pb):
+    fl = ((pb.flags & 1) != 0) & (pb.inv_mass > 0)
+    return np.nonzero(fl)[0]
+
+
+def test_c2_dam_break_1m_one_step():
+    cfg = scenes.DAM_BREAK_1M
+    _print("C2: dam break 1,048,576 fluid particles, one step vs FP32 restatement / FP64 oracle")
+    m = scenes.build_dam_break(cfg).finalize(DEV)
+    s = SolverSPH(m, cfg.support, warm_start="prev")
+    a, b = m.state(), m.state()
+    s.step(a, b, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    rep = s.last_report
+    fld = s.fields()
+    q, qd, f = _np(b.particle_q), _np(b.particle_qd), _np(s.sph_force)
+    rho, p, cnt = _np(fld["density"]), _np(fld["pressure"]), _np(fld["neighbor_count"])
+    pb = O.problem_from_model(m, s)
+    r = O.step(pb, pb.q0, pb.qd0, precision="f32")
+    _print(f"  iterations {rep['max_iterations']} (restatement {int(r['iters'][0])}), "
+           f"max neighbours {rep['max_neighbor_count']}")
+    assert rep["unconverged_worlds"] == 0 and r["status"] == 0
+    assert int(r["iters"][0]) == rep["max_iterations"]
+    for name, g, o in (("x", q, r["q"]), ("v", qd, r["qd"]), ("f", f, r["f"]), ("rho", rho, r["rho"]),
+                       ("p", p, r["p"])):
+        assert np.array_equal(g, o), f"{name} differs from the FP32 restatement"
+    r64 = O.step(pb, pb.q0, pb.qd0, precision="f64", forced_iters=np.array([rep["max_iterations"]]))
+    fl = rho > 0
+    drho = np.abs(rho[fl] - r64["rho"][fl]) / r64["rho"][fl]
+    fmag = np.abs(r64["f"][fl]).max()
+    df = np.abs(f[fl].astype(np.float64) - r64["f"][fl]).max() / fmag
+    _print(f"  vs FP64: max rel density error {drho.max():.2e}, max force error / max |F| {df:.2e}")
+    assert drho.max() <= REL and df <= REL
+    # neighbour lists, slot for slot (particle ids, ascending cell-sorted order)
+    counts, idx = s.neighbors()
+    oc, oi, st = O.solver_neighbors(pb, pb.q0)
+    assert st == 0
+    assert np.array_equal(_np(counts), oc) and np.array_equal(_np(counts), cnt)
+    k = int(oc.max())
+    assert np.array_equal(_np(idx[:k]), oi[:k])
+    _print(f"  neighbour lists equal ({int(oc.sum())} entries, longest {k})")
+
+
+def _sample_worlds(W: int, n: int, seed: int) -> list[int]:
+    rng = np.random.default_rng(seed)
+    return sorted(rng.choice(W, size=n, replace=False).tolist())
+
+
+def _check_world_samples(m, s, H, b_state, worlds, warm: str, p0_batch=None, mlp=None, dt=scenes.DT):
+    """Each sampled world alone on the GPU equals its part of the batch; the FP32 restatement
+    (from the batch's warm-start pressure when given) equals it too."""
+    q, qd = _np(b_state.particle_q), _np(b_state.particle_qd)
+    world = _np(m.particle_world)
+    its = s.world_stats()["iterations"]
+    for w in worlds:
+        mw = scenes.extract_world(m, w).finalize(DEV)
+        sw = SolverSPH(mw, H[w], warm_start=warm if warm != "mlp" else "zero")
+        if mlp is not None:
+            sw.set_mlp(mlp)
+        a, c = mw.state(), mw.state()
+        sw.step(a, c, None, None, dt)
+        torch.cuda.synchronize()
+        sel = world == w
+        assert np.array_equal(_np(c.particle_q), q[sel]), f"world {w}: alone != batch"
+        assert np.array_equal(_np(c.particle_qd), qd[sel]), f"world {w}: alone != batch"
+        assert int(sw.world_stats()["iterations"][0]) == int(its[w]), w
+        pb = O.problem_from_model(mw, sw)
+        pprev = None
+        if p0_batch is not None:
+            pb.warm_start = 2  # the restatement starts from the given p0
+            pprev = p0_batch[sel][_fluid_order(pb)]
+        r = O.step(pb, pb.q0, pb.qd0, pprev=pprev, precision="f32", dt=dt)
+        assert np.array_equal(r["q"], q[sel]), f"world {w}: FP32 restatement differs"
+        assert np.array_equal(r["qd"], qd[sel]), f"world {w}: FP32 restatement differs"
+        assert int(r["iters"][0]) == int(its[w]), w
+
+
+def test_c3_batched_4096_one_step():
+    cfg = scenes.BATCH_4096
+    _print("C3: 4096 worlds, one step; 16 sampled worlds alone and vs the FP32 restatement")
+    b, H = scenes.build_batched(cfg)
+    m = b.finalize(DEV)
+    s = SolverSPH(m, H, warm_start="prev")
+    a, c = m.state(), m.state()
+    s.step(a, c, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    ws = s.world_stats()
+    _print(f"  {s.fluid_count} fluid particles; iterations {int(ws['iterations'].min())}.."
+           f"{int(ws['iterations'].max())}; unconverged {s.last_report['unconverged_worlds']}")
+    assert s.last_report["unconverged_worlds"] == 0 and (ws["state"] == 1).all()
+    assert np.isfinite(_np(c.particle_q)).all()
+    _check_world_samples(m, s, H, c, _sample_worlds(cfg.n_worlds, 16, 2024), "prev")
+
+
+def test_c4_learned_warm_start_1m_converges():
+    cfg = scenes.DAM_BREAK_1M
+    _print("C4: shipped network as warm start on the 1M dam break, 12 steps")
+    m = scenes.build_dam_break(cfg).finalize(DEV)
+    s = SolverSPH(m, cfg.support)
+    s.set_mlp(WarmStartMLP.load(str(MLP_PATH)))
+    a, b = m.state(), m.state()
+    its = []
+    for k in range(12):
+        s.step(a, b, None, None, scenes.DT)
+        rep = s.last_report
+        its.append(rep["max_iterations"])
+        assert rep["unconverged_worlds"] == 0 and rep["max_density_error"] <= s.tolerance, (k, rep)
+        if k == 10:
+            # the step from the network's prediction equals the restatement from that p0
+            p0 = _np(s.warm_pressure())
+            q_in, qd_in = _np(a.particle_q), _np(a.particle_qd)
+            torch.cuda.synchronize()
+            q_gpu, qd_gpu = _np(b.particle_q), _np(b.particle_qd)
+        a, b = b, a
+    _print(f"  iterations per step {its}")
+    assert float(np.abs(p0).max()) > 0.0
+    pb = O.problem_from_model(m, s)
+    pb.warm_start = 2
+    r = O.step(pb, q_in, qd_in, pprev=p0[_fluid_order(pb)], precision="f32")
+    assert np.array_equal(r["q"], q_gpu) and np.array_equal(r["qd"], qd_gpu)
+    assert int(r["iters"][0]) == its[10]
+
+
+def test_c5_batched_1024_mlp_graph():
+    cfg = scenes.BATCH_1024
+    _print("C5: 1024 worlds (4096..65536), learned warm start, 3 graph replays vs 3 eager steps")
+    b, H = scenes.build_batched(cfg)
+    m = b.finalize(DEV)
+    mlp = WarmStartMLP.load(str(MLP_PATH))
+    se = SolverSPH(m, H)
+    se.set_mlp(mlp)
+    sg = SolverSPH(m, H, use_graph=True, graph_slots=3)
+    sg.set_mlp(mlp)
+    ea, eb = m.state(), m.state()
+    ga, gb = m.state(), m.state()
+    for k in range(3):
+        se.step(ea, eb, None, None, scenes.DT)
+        if k == 0:
+            p0_eager = _np(se.warm_pressure())
+        sg.step(ga, gb, None, None, scenes.DT)
+        torch.cuda.synchronize()
+        assert se.last_report["unconverged_worlds"] == 0
+        assert torch.equal(eb.particle_q, gb.particle_q) and torch.equal(eb.particle_qd, gb.particle_qd), k
+        assert torch.equal(eb.sph.pressure, gb.sph.pressure), k
+        if k == 0:
+            first = (eb.particle_q.clone(), eb.particle_qd.clone())
+        ea, eb = eb, ea
+        ga, gb = gb, ga
+    _print(f"  {se.fluid_count} fluid particles; graph replays equal eager steps")
+    del sg, ga, gb, ea, eb
+    # sampled worlds of the first step: alone (GPU, same network) and the FP32 restatement
+    # from the batch's warm-start pressure; the first step is re-run to have its per-world stats
+    se0 = SolverSPH(m, H)
+    se0.set_mlp(mlp)
+    a0, c0 = m.state(), m.state()
+    se0.step(a0, c0, None, None, scenes.DT)
+    torch.cuda.synchronize()
+    assert torch.equal(c0.particle_q, first[0])
+    assert np.array_equal(_np(se0.warm_pressure()), p0_eager)
+    _check_world_samples(m, se0, H, c0, _sample_worlds(cfg.n_worlds, 4, 77), "mlp", p0_batch=p0_eager, mlp=mlp)

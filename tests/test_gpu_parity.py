@@ -98,7 +98,7 @@ def test_step_small_dam_break_parity(dims):
     f = s.fields()
     counts, idx = s.neighbors()
     g = {k: v.cpu().numpy() for k, v in f.items()}
-    q1, qd1, f1 = (t.cpu().numpy() for t in (st1.particle_q, st1.particle_qd, st1.particle_f))
+    q1, qd1, f1 = (t.cpu().numpy() for t in (st1.particle_q, st1.particle_qd, s.sph_force))
     counts, idx = counts.cpu().numpy(), idx.cpu().numpy()
 
     pb = O.problem_from_model(m, s)
@@ -255,7 +255,7 @@ def test_dam_break_32k_one_step():
     torch.cuda.synchronize()
     rep = s.last_report
     g = {k: v.cpu().numpy() for k, v in s.fields().items()}
-    f1 = st1.particle_f.cpu().numpy()
+    f1 = s.sph_force.cpu().numpy()
     q1 = st1.particle_q.cpu().numpy()
     pb = O.problem_from_model(m, s)
     r32 = O.step(pb, pb.q0, pb.qd0, precision="f32")

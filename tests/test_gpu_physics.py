@@ -26,7 +26,7 @@ def test_hydrostatic_column():
         iters.append(s.last_report["max_iterations"])
         a, b = b, a
     torch.cuda.synchronize()
-    q, qd, f = (t.cpu().numpy() for t in (a.particle_q, a.particle_qd, a.particle_f))
+    q, qd, f = (t.cpu().numpy() for t in (a.particle_q, a.particle_qd, s.sph_force))
     fields = {k: v.cpu().numpy() for k, v in s.fields().items()}
     pb = O.problem_from_model(m, s)
     mt = HC.metrics(q, qd, fields["pressure"], fields["density"], f, pb.mass, pb.inv_mass > 0)

@@ -149,6 +149,62 @@ def test_xpbd_and_penalty_consumers_match_oracle():
     assert np.allclose(_np(f), fr, rtol=0, atol=2e-5 * np.abs(fr).max())
 
 
+def test_xpbd_step_with_shape_contacts_matches_oracle():
+    """SolverXPBD.step with particle-shape contacts: positional projection only
+    (solver_xpbd.py:386-418), no penalty force, state_in.particle_f only read.  The oracle
+    projects the contact list Model.collide produced (its generation is checked above)."""
+    from oracle import contacts_oracle as C
+
+    rng = np.random.default_rng(11)
+    b = ModelBuilder()
+    r = 0.05
+    # two well separated layers: one dipping into the plane, one under the sphere; no
+    # projection can bring two particles within contact distance (the grid is built before
+    # the projections, solver_xpbd.py:315-319, so pairs a projection creates across cells are
+    # not candidates in the reference either; the brute-force oracle would see them)
+    g = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(2), indexing="ij"), -1).reshape(-1, 3)
+    x = (g * np.array([0.3, 0.3, 0.27]) + np.array([-0.75, -0.75, -0.02])) + rng.uniform(-0.01, 0.01, g.shape)
+    n = x.shape[0]
+    b.add_particles(x.tolist(), (rng.normal(0.0, 0.5, (n, 3))).tolist(), [1.0] * n, [r] * n)
+    b.add_ground_plane(height=0.0)
+    b.add_shape_sphere(-1, ((0.15, 0.15, 0.33), (0.0, 0.0, 0.0, 1.0)), radius=0.12)
+    m = b.finalize(DEV)
+    m.particle_mu = 0.3
+    sol = SolverXPBD(m, iterations=3)
+    s0, s1 = m.state(), m.state()
+    s0.particle_f.normal_(0.0, 0.05, generator=torch.Generator(DEV).manual_seed(3))
+    contacts = m.contacts()
+    m.collide(s0, contacts, soft_contact_margin=0.01)
+    x0, v0, f0 = _np(s0.particle_q), _np(s0.particle_qd), _np(s0.particle_f)
+    flags = m.particle_flags.cpu().numpy()
+    cnt = int(contacts.soft_contact_count.item())
+    cl = [(int(i), int(sh), bp, bv, nr) for i, sh, bp, bv, nr in zip(
+        contacts.soft_contact_particle[:cnt].cpu().numpy(), contacts.soft_contact_shape[:cnt].cpu().numpy(),
+        _np(contacts.soft_contact_body_pos[:cnt]), _np(contacts.soft_contact_body_vel[:cnt]),
+        _np(contacts.soft_contact_normal[:cnt]))]
+    _print(f"SolverXPBD, {n} particles on a plane and under a sphere, {cnt} shape contacts")
+    assert cnt >= 36 and any(c[1] == 1 for c in cl)
+    dt = 1.0 / 240.0
+    f_before = s0.particle_f.clone()
+    sol.step(s0, s1, None, contacts, dt)
+    torch.cuda.synchronize()
+    assert torch.equal(s0.particle_f, f_before), "SolverXPBD must not write state_in.particle_f"
+    inv_mass = m.particle_inv_mass.cpu().numpy().astype(np.float64)
+    radius = m.particle_radius.cpu().numpy().astype(np.float64)
+    xr, vr = C.xpbd_step(x0, v0, f0, inv_mass, radius, flags, None, _np(m.gravity).reshape(-1, 3), dt,
+                         m.particle_max_velocity, 0.3, m.particle_cohesion, 3, shape_contacts=cl,
+                         shape_mu=_np(m.shape_material_mu), soft_contact_mu=float(m.soft_contact_mu),
+                         particle_adhesion=float(m.particle_adhesion))
+    e = np.abs(_np(s1.particle_q) - xr).max(axis=1)
+    i = int(np.argmax(e))
+    _print(f"  max |x - oracle| {e[i]:.2e} at particle {i} (x0 {x0[i]}, contacts "
+           f"{sum(1 for c in cl if c[0] == i)}, gpu {_np(s1.particle_q)[i]}, oracle {xr[i]})")
+    assert e[i] < 2e-6
+    assert np.abs(_np(s1.particle_qd) - vr).max() < 2e-6 / dt
+    # the projection moves every particle that touched the plane out of it (relaxation 0.9)
+    assert _np(s1.particle_q)[:, 2].min() > 0.5 * r
+
+
 @pytest.mark.parametrize("solver", ["xpbd", "semi_implicit"])
 def test_particle_comes_to_rest_on_ground_plane(solver):
     _print(f"{solver}: particle dropped on a ground plane")

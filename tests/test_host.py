@@ -128,7 +128,7 @@ def test_abi_exports_every_declared_symbol():
     assert len(names) >= 25
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.nsph_abi_version() == 1
+    assert lib.nsph_abi_version() == 2
     # every declared function has a ctypes signature in the binding
     for name in names:
         assert name in N.SIGNATURES or name in ("nsph_last_error", "nsph_kernel_name"), name
@@ -148,7 +148,7 @@ def test_abi_errors_without_device():
     st = lib.nsph_ctx_create(0, None, C.byref(out))
     assert st != 0
     assert len(N.last_error()) > 0
-    assert lib.nsph_solver_step(None, None, None, None, None, None, C.c_float(1e-4), 0, None) == 1
+    assert lib.nsph_solver_step(None, None, None, C.c_float(1e-4), 0, None) == 1
 
 
 def test_bench_accounts_every_kernel_class():
