@@ -115,12 +115,17 @@ int nsph_eval_particle_contact(nsph_grid* grid, int n, const float* d_x, const f
 
 /* ---- particle-shape soft contacts, static shapes (body = -1) ------------------------ */
 
+/* Workspace of nsph_soft_contacts for n particles, in ints: a count per particle and one per
+ * workgroup of 256 particles. */
+#define NSPH_SOFT_WORK_INTS(n) ((n) + ((n) + 255) / 256)
+
 /* create_soft_contacts (newton/_src/geometry/kernels.py:1000-1141, launched at
  * collide.py:1194-1230) for plane, sphere, capsule, cylinder and box shapes.  Shapes are
  * d_shape_xform[7 s] = (px, py, pz, qx, qy, qz, qw), d_shape_scale[3 s], type (GeoType
  * values), world and flags (ShapeFlags).  Contacts are ordered by (particle, shape);
  * d_start[n + 1] receives each particle's first contact slot, *d_count the total (which may
- * exceed `capacity`; slots at or beyond it are not written).  d_work: n ints of scratch. */
+ * exceed `capacity`; slots at or beyond it are not written).  d_work: NSPH_SOFT_WORK_INTS(n)
+ * ints of scratch. */
 int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
                        const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
                        const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,

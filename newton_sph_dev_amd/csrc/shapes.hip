@@ -161,38 +161,45 @@ __device__ inline void particle_shape_contacts(const ShapeView& S, V3 x, float r
     }
 }
 
-__global__ void k_soft_count(int n, const float* __restrict__ x, const float* __restrict__ radius,
-                             const int* __restrict__ flags, const int* __restrict__ world, ShapeView S,
-                             float margin, int* __restrict__ counts) {
+// Contacts per particle (work[i]) and per workgroup of 256 particles (bsum[block]).
+__global__ __launch_bounds__(256) void k_soft_count(int n, const float* __restrict__ x,
+                                                    const float* __restrict__ radius, const int* __restrict__ flags,
+                                                    const int* __restrict__ world, ShapeView S, float margin,
+                                                    int* __restrict__ counts, int* __restrict__ bsum) {
+    __shared__ int wsum[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     int c = 0;
-    if ((flags[i] & NSPH_PARTICLE_ACTIVE) != 0)
+    if (i < n && (flags[i] & NSPH_PARTICLE_ACTIVE) != 0)
         particle_shape_contacts(S, ld3(x, i), radius[i], world ? world[i] : -1, margin,
                                 [&](int, V3, float, V3, V3, float4) { ++c; });
-    counts[i] = c;
+    if (i < n) counts[i] = c;
+    int v = c;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
-// Exclusive scan of counts[0..n) into start[0..n] (start[n] = total) with one workgroup of
-// 256 threads walking the array in tiles; totals fit in int for the capacities used here.
-__global__ void k_soft_scan(int n, const int* __restrict__ counts, int* __restrict__ start,
-                            int* __restrict__ total_out) {
-    __shared__ int sh[256];
+// Exclusive scan of the nb workgroup sums in place (one workgroup of 1024 threads walking the
+// array in tiles of 4096); start[n] and *total_out receive the total.
+__global__ __launch_bounds__(1024) void k_soft_scan_blocks(int nb, int* __restrict__ bsum, int n,
+                                                           int* __restrict__ start, int* __restrict__ total_out) {
+    __shared__ int sh[1024];
     __shared__ int carry;
     const int t = threadIdx.x;
     if (t == 0) carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 256 * 4) {
+    for (int base = 0; base < nb; base += 1024 * 4) {
         int v[4], sum = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int i = base + t * 4 + k;
-            v[k] = i < n ? counts[i] : 0;
+            v[k] = i < nb ? bsum[i] : 0;
             sum += v[k];
         }
         sh[t] = sum;
         __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
+        for (int off = 1; off < 1024; off <<= 1) {
             const int y = t >= off ? sh[t - off] : 0;
             __syncthreads();
             sh[t] += y;
@@ -202,11 +209,11 @@ __global__ void k_soft_scan(int n, const int* __restrict__ counts, int* __restri
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int i = base + t * 4 + k;
-            if (i < n) start[i] = run;
+            if (i < nb) bsum[i] = run;
             run += v[k];
         }
         __syncthreads();
-        if (t == 255) carry += sh[255];
+        if (t == 1023) carry += sh[1023];
         __syncthreads();
     }
     if (t == 0) {
@@ -215,15 +222,31 @@ __global__ void k_soft_scan(int n, const int* __restrict__ counts, int* __restri
     }
 }
 
-__global__ void k_soft_write(int n, const float* __restrict__ x, const float* __restrict__ radius,
-                             const int* __restrict__ flags, const int* __restrict__ world, ShapeView S,
-                             float margin, const int* __restrict__ start, int cap, int* __restrict__ c_particle,
-                             int* __restrict__ c_shape, float* __restrict__ c_body_pos,
-                             float* __restrict__ c_body_vel, float* __restrict__ c_normal) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// First contact slot of each particle (its workgroup's offset plus the exclusive scan of the
+// workgroup's counts), then the contacts themselves.
+__global__ __launch_bounds__(256) void k_soft_write(int n, const float* __restrict__ x,
+                                                    const float* __restrict__ radius, const int* __restrict__ flags,
+                                                    const int* __restrict__ world, ShapeView S, float margin,
+                                                    const int* __restrict__ counts, const int* __restrict__ boff,
+                                                    int* __restrict__ start, int cap, int* __restrict__ c_particle,
+                                                    int* __restrict__ c_shape, float* __restrict__ c_body_pos,
+                                                    float* __restrict__ c_body_vel, float* __restrict__ c_normal) {
+    __shared__ int sh[256];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + t;
+    const int c = i < n ? counts[i] : 0;
+    sh[t] = c;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int y = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += y;
+        __syncthreads();
+    }
     if (i >= n) return;
+    int k = boff[blockIdx.x] + sh[t] - c;
+    start[i] = k;
     if ((flags[i] & NSPH_PARTICLE_ACTIVE) == 0) return;
-    int k = start[i];
     particle_shape_contacts(S, ld3(x, i), radius[i], world ? world[i] : -1, margin,
                             [&](int s, V3 local, float d, V3 nrm, V3 pos, float4 q) {
                                 if (k < cap) {
@@ -330,14 +353,11 @@ using namespace nsph;
 
 extern "C" {
 
-int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
-                       const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
-                       const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
-                       float margin, int capacity, int* d_work, int* d_start, int* d_count, int* d_particle,
-                       int* d_shape, float* d_body_pos, float* d_body_vel, float* d_normal) {
-    NSPH_CHECK(ctx && n >= 0 && n_shapes >= 0 && capacity >= 0, NSPH_ERR_INVALID,
-               "nsph_soft_contacts: bad argument");
-    NSPH_CHECK(d_start && d_count && d_work, NSPH_ERR_INVALID, "nsph_soft_contacts: NULL output");
+static int soft_contacts(const char* who, nsph_ctx* ctx, int n, const float* d_x, const float* d_radius,
+                         const int* d_flags, const int* d_world, int n_shapes, const float* d_shape_xform,
+                         const int* d_shape_type, const float* d_shape_scale, const int* d_shape_world,
+                         const int* d_shape_flags, float margin, int capacity, int* d_work, int* d_start, int* d_count,
+                         int* d_particle, int* d_shape, float* d_body_pos, float* d_body_vel, float* d_normal) {
     hipStream_t st = ctx->stream;
     if (n == 0 || n_shapes == 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3((n + 256) / 256 + 1), dim3(256), 0, st, d_start, (size_t)n + 1, 0);
@@ -347,15 +367,31 @@ int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_ra
     }
     NSPH_CHECK(d_x && d_radius && d_flags && d_shape_xform && d_shape_type && d_shape_scale && d_shape_world &&
                    d_shape_flags && d_particle && d_shape && d_body_pos && d_body_vel && d_normal,
-               NSPH_ERR_INVALID, "nsph_soft_contacts: NULL array");
+               NSPH_ERR_INVALID, "%s: NULL array", who);
     ShapeView S{d_shape_xform, d_shape_type, d_shape_scale, d_shape_world, d_shape_flags, n_shapes};
-    const dim3 g((n + 255) / 256);
-    hipLaunchKernelGGL(k_soft_count, g, dim3(256), 0, st, n, d_x, d_radius, d_flags, d_world, S, margin, d_work);
-    hipLaunchKernelGGL(k_soft_scan, dim3(1), dim3(256), 0, st, n, d_work, d_start, d_count);
-    hipLaunchKernelGGL(k_soft_write, g, dim3(256), 0, st, n, d_x, d_radius, d_flags, d_world, S, margin, d_start,
-                       capacity, d_particle, d_shape, d_body_pos, d_body_vel, d_normal);
+    const int nb = (n + 255) / 256;
+    int* counts = d_work;
+    int* bsum = d_work + n;
+    hipLaunchKernelGGL(k_soft_count, dim3(nb), dim3(256), 0, st, n, d_x, d_radius, d_flags, d_world, S, margin,
+                       counts, bsum);
+    hipLaunchKernelGGL(k_soft_scan_blocks, dim3(1), dim3(1024), 0, st, nb, bsum, n, d_start, d_count);
+    hipLaunchKernelGGL(k_soft_write, dim3(nb), dim3(256), 0, st, n, d_x, d_radius, d_flags, d_world, S, margin,
+                       counts, bsum, d_start, capacity, d_particle, d_shape, d_body_pos, d_body_vel, d_normal);
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;
+}
+
+int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
+                       const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
+                       const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
+                       float margin, int capacity, int* d_work, int* d_start, int* d_count, int* d_particle,
+                       int* d_shape, float* d_body_pos, float* d_body_vel, float* d_normal) {
+    NSPH_CHECK(ctx && n >= 0 && n_shapes >= 0 && capacity >= 0, NSPH_ERR_INVALID,
+               "nsph_soft_contacts: bad argument");
+    NSPH_CHECK(d_start && d_count && d_work, NSPH_ERR_INVALID, "nsph_soft_contacts: NULL output");
+    return soft_contacts("nsph_soft_contacts", ctx, n, d_x, d_radius, d_flags, d_world, n_shapes, d_shape_xform,
+                         d_shape_type, d_shape_scale, d_shape_world, d_shape_flags, margin, capacity, d_work, d_start,
+                         d_count, d_particle, d_shape, d_body_pos, d_body_vel, d_normal);
 }
 
 int nsph_xpbd_shape_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_v, const float* d_inv_mass,

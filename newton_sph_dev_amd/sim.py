@@ -764,7 +764,9 @@ class Contacts:
         self.soft_contact_body_vel = torch.zeros((m, 3), dtype=torch.float32, device=dev)
         self.soft_contact_normal = torch.zeros((m, 3), dtype=torch.float32, device=dev)
         self.soft_contact_start = torch.zeros(particle_count + 1, dtype=torch.int32, device=dev)
-        self._work = torch.zeros(max(particle_count, 1), dtype=torch.int32, device=dev)
+        # nsph_soft_contacts' workspace: a count per particle and one per 256 particles
+        self._work = torch.zeros(max(particle_count + (particle_count + 255) // 256, 1), dtype=torch.int32,
+                                 device=dev)
 
     def clear(self) -> None:
         self.soft_contact_count.zero_()
