@@ -131,6 +131,20 @@ int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_ra
                        const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
                        float margin, int capacity, int* d_work, int* d_start, int* d_count, int* d_particle,
                        int* d_shape, float* d_body_pos, float* d_body_vel, float* d_normal);
+/* nsph_soft_contacts without a workspace argument and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The workspace is a
+ * library-owned device array of NSPH_SOFT_WORK_INTS(NSPH_SOFT_STATIC_MAX_PARTICLES) ints
+ * (about 16 MB), one per GPU, resolved by nsph_ctx_create and held until the process ends;
+ * n above NSPH_SOFT_STATIC_MAX_PARTICLES fails with NSPH_ERR_CAPACITY.  Every call on a GPU
+ * shares that GPU's array (each call writes all of the workspace it reads), so calls that
+ * overlap on one GPU (different streams or threads) race.  Results equal nsph_soft_contacts'. */
+#define NSPH_SOFT_STATIC_MAX_PARTICLES 4000000
+int nsph_soft_contacts_static(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
+                              const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
+                              const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
+                              float margin, int capacity, int* d_start, int* d_count, int* d_particle, int* d_shape,
+                              float* d_body_pos, float* d_body_vel, float* d_normal);
 /* solve_particle_shape_contacts (newton/_src/solvers/xpbd/kernels.py:114-222, launched at
  * solver_xpbd.py:386-418) for static shapes: adds each active particle's corrections over
  * its contacts into d_deltas[i]. */

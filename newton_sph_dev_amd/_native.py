@@ -146,6 +146,8 @@ SIGNATURES: dict[str, list] = {
     "nsph_eval_particle_contact": [_vp, _i, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _f, _vp],
     "nsph_soft_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp, _vp, _vp,
                            _vp, _vp, _vp, _vp],
+    "nsph_soft_contacts_static": [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _f, _i, _vp, _vp, _vp,
+                                  _vp, _vp, _vp, _vp],
     "nsph_xpbd_shape_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp, _i, _vp, _vp, _vp, _vp, _f,
                                  _f, _vp],
     "nsph_eval_shape_contacts": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _f, _vp, _i, _vp,

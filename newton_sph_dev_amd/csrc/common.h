@@ -14,6 +14,9 @@ namespace nsph {
 
 void set_error(const char* fmt, ...);
 
+// Resolve the static soft-contact workspace of `device` (current); called by nsph_ctx_create.
+int soft_static_init(int device);
+
 #define NSPH_HIP(call)                                                                     \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

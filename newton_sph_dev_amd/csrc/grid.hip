@@ -185,6 +185,7 @@ int nsph_ctx_create(int device, void* stream, nsph_ctx** out) {
     NSPH_CHECK(device >= 0 && device < count, NSPH_ERR_INVALID,
                "nsph_ctx_create: device %d out of range (%d devices)", device, count);
     NSPH_HIP(hipSetDevice(device));
+    NSPH_TRY(soft_static_init(device));
     nsph_ctx* c = new nsph_ctx();
     c->device = device;
     c->stream = (hipStream_t)stream;

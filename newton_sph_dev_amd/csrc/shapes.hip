@@ -15,6 +15,8 @@
 
 #include <cmath>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace nsph {
@@ -381,6 +383,33 @@ static int soft_contacts(const char* who, nsph_ctx* ctx, int n, const float* d_x
     return NSPH_OK;
 }
 
+}  // extern "C"
+
+// Workspace of nsph_soft_contacts_static: a file-scope device array, one copy per GPU, held
+// until the process ends.  Host code never passes the array's name to a kernel (on the host it
+// is only a stand-in for the symbol): nsph_soft_static_init resolves its device address once
+// per device, with that device current, and every use goes through that address.
+static __device__ __align__(256) int g_soft_scratch[NSPH_SOFT_WORK_INTS(NSPH_SOFT_STATIC_MAX_PARTICLES)];
+static constexpr int kMaxDevices = 64;
+static int* g_soft_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_soft_scratch_mu;
+
+namespace nsph {
+int soft_static_init(int device) {
+    NSPH_CHECK(device >= 0 && device < kMaxDevices, NSPH_ERR_INVALID, "soft-contact scratch: device %d out of range",
+               device);
+    std::lock_guard<std::mutex> lock(g_soft_scratch_mu);
+    if (g_soft_scratch_ptr[device]) return NSPH_OK;
+    int* p = nullptr;
+    NSPH_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(g_soft_scratch)));
+    NSPH_CHECK(p != nullptr, NSPH_ERR_HIP, "soft-contact scratch: no device address");
+    g_soft_scratch_ptr[device] = p;
+    return NSPH_OK;
+}
+}  // namespace nsph
+
+extern "C" {
+
 int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
                        const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
                        const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
@@ -392,6 +421,26 @@ int nsph_soft_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_ra
     return soft_contacts("nsph_soft_contacts", ctx, n, d_x, d_radius, d_flags, d_world, n_shapes, d_shape_xform,
                          d_shape_type, d_shape_scale, d_shape_world, d_shape_flags, margin, capacity, d_work, d_start,
                          d_count, d_particle, d_shape, d_body_pos, d_body_vel, d_normal);
+}
+
+int nsph_soft_contacts_static(nsph_ctx* ctx, int n, const float* d_x, const float* d_radius, const int* d_flags,
+                              const int* d_world, int n_shapes, const float* d_shape_xform, const int* d_shape_type,
+                              const float* d_shape_scale, const int* d_shape_world, const int* d_shape_flags,
+                              float margin, int capacity, int* d_start, int* d_count, int* d_particle, int* d_shape,
+                              float* d_body_pos, float* d_body_vel, float* d_normal) {
+    NSPH_CHECK(ctx && n >= 0 && n_shapes >= 0 && capacity >= 0, NSPH_ERR_INVALID,
+               "nsph_soft_contacts_static: bad argument");
+    NSPH_CHECK(d_start && d_count, NSPH_ERR_INVALID, "nsph_soft_contacts_static: NULL output");
+    NSPH_CHECK(n <= NSPH_SOFT_STATIC_MAX_PARTICLES, NSPH_ERR_CAPACITY,
+               "nsph_soft_contacts_static: %d particles exceed the static workspace (at most %d)", n,
+               NSPH_SOFT_STATIC_MAX_PARTICLES);
+    NSPH_CHECK(ctx->device >= 0 && ctx->device < kMaxDevices && g_soft_scratch_ptr[ctx->device], NSPH_ERR_INVALID,
+               "nsph_soft_contacts_static: the static workspace of device %d is not set up (nsph_ctx_create)",
+               ctx->device);
+    return soft_contacts("nsph_soft_contacts_static", ctx, n, d_x, d_radius, d_flags, d_world, n_shapes,
+                         d_shape_xform, d_shape_type, d_shape_scale, d_shape_world, d_shape_flags, margin, capacity,
+                         g_soft_scratch_ptr[ctx->device], d_start, d_count, d_particle, d_shape, d_body_pos,
+                         d_body_vel, d_normal);
 }
 
 int nsph_xpbd_shape_contacts(nsph_ctx* ctx, int n, const float* d_x, const float* d_v, const float* d_inv_mass,
