@@ -145,6 +145,10 @@ struct SphDev {
     int* act;
     int* flags;  // [0] overflow, [2] non-finite
     int* wmax;   // per wave of the neighbour kernel: its largest uncapped count
+    int* bq_count;   // [0]: particles queued for the boundary visit this step
+    int2* bq_id;     // queued (sorted index, world)
+    float4* bq_a;    // their fluid density state: (sum, compensation, Gf.x, Gf.y)
+    float2* bq_b;    //                            (Gf.z, S)
     int nwave;   // 4 per chunk
 };
 
@@ -336,6 +340,7 @@ __global__ void k_sph_begin(SphDev D, int W, int n, const uint32_t* prev_keys, i
         D.act[1] = W;
         D.flags[0] = 0;
         D.flags[2] = 0;
+        D.bq_count[0] = 0;
     }
     if (k < D.nwave) D.wmax[k] = 0;
     if (k < n) {
@@ -665,6 +670,21 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 //            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
 // vs[s].w so that the viscosity pass fetches velocity and density together.
+// End of the density pass for sorted particle s: rho = (fluid + boundary compensated sum),
+// G = Gf + psi Gb, a_ii = -dt^2 ((G.G + m_i S) / rho^2), and the per-particle outputs.
+__device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s, float mi, float sum, float cmp,
+                                      float gfx, float gfy, float gfz, float S, float gbx, float gby, float gbz,
+                                      float mr2, int nb, float dt2) {
+    const float rho = sum + cmp;
+    const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
+    const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
+    D.rho[s] = rho;
+    reinterpret_cast<float*>(D.vs + s)[3] = rho;
+    D.aux[s] = make_float2((float)nb, nb > 0 ? sqrtf(mr2) * P.invH : 1.0f);
+    D.aii[s] = -(dt2 * (num / (rho * rho)));
+    D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
+}
+
 #define NBR_LDS_QUADS 6
 // Six waves per SIMD (80 VGPRs, no scratch; the LDS staging allows six workgroups of 256
 // per CU) hide more of the gathers' latency than the five the register count gives unasked.
@@ -749,9 +769,44 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
             S += xj.w * dot3(gx, gy, gz, gx, gy, gz);
         });
 
-    float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
-    int nb = 0;
-    if (D.breach[g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx]) {
+    // Particles whose cell has boundary particles in its 3x3x3 block are queued with their
+    // fluid sums; k_sph_bnd_density adds the boundary terms in full waves (the visit is
+    // per-lane and a wave holds few such particles).  The others finish here.
+    const bool breach = D.breach[g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx] != 0;
+    const uint64_t bm = __ballot(breach);
+    if (bm) {
+        const int lane = (int)__lane_id();
+        const int leader = __ffsll((unsigned long long)bm) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(D.bq_count, __popcll(bm));
+        base = __shfl(base, leader, 64);
+        if (breach) {
+            const int slot = base + __popcll(bm & ((1ull << lane) - 1ull));
+            D.bq_id[slot] = make_int2(s, w);
+            D.bq_a[slot] = make_float4(sum, cmp, gfx, gfy);
+            D.bq_b[slot] = make_float2(gfz, S);
+            return;
+        }
+    }
+    density_finish(D, P, s, xi.w, sum, cmp, gfx, gfy, gfz, S, 0.f, 0.f, 0.f, P.H2, 0, dt2);
+}
+
+// Boundary terms of the queued particles, continuing their compensated density sum in the
+// boundary candidate order, then the same finish as the neighbour kernel.  Grid-stride over
+// the queue (its length is known on the device only).
+__global__ __launch_bounds__(256) void k_sph_bnd_density(SphDev D, float dt2) {
+    const int n = *D.bq_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 id = D.bq_id[i];
+        const int s = id.x, w = id.y;
+        const GridWorld g = D.gw[w];
+        const PhysWorld P = D.phys[w];
+        const float4 xi = D.xs[s];
+        const float4 a = D.bq_a[i];
+        const float2 b = D.bq_b[i];
+        float sum = a.x, cmp = a.y;
+        float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
+        int nb = 0;
         rows_visit(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                    [&](int, float dx, float dy, float dz, float r2) {
                        float W, gf;
@@ -763,15 +818,8 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
                        mr2 = fminf(mr2, r2);
                        ++nb;
                    });
+        density_finish(D, P, s, xi.w, sum, cmp, a.z, a.w, b.x, b.y, gbx, gby, gbz, mr2, nb, dt2);
     }
-    const float rho = sum + cmp;
-    const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
-    const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + xi.w * S;
-    D.rho[s] = rho;
-    reinterpret_cast<float*>(D.vs + s)[3] = rho;
-    D.aux[s] = make_float2((float)nb, nb > 0 ? sqrtf(mr2) * P.invH : 1.0f);
-    D.aii[s] = -(dt2 * (num / (rho * rho)));
-    D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
 }
 
 struct XV {
@@ -1470,7 +1518,7 @@ struct nsph_solver {
     uint32_t* perm = nullptr;
     RadixSort rs;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, p0, pprev, nbr, ncnt, aux;
-    DevBuf partial, iters, err, conv, act, flags, wmax;
+    DevBuf partial, iters, err, conv, act, flags, wmax, bq_count, bq_id, bq_a, bq_b;
     int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
     RepDev* d_rep_host = nullptr;  // device address of the pinned RepDev
     int rep_seq = 0;               // sequence number of the last step summary
@@ -1526,6 +1574,10 @@ static SphDev make_dev(nsph_solver* S) {
     D.act = bp<int>(S->act);
     D.flags = bp<int>(S->flags);
     D.wmax = bp<int>(S->wmax);
+    D.bq_count = bp<int>(S->bq_count);
+    D.bq_id = bp<int2>(S->bq_id);
+    D.bq_a = bp<float4>(S->bq_a);
+    D.bq_b = bp<float2>(S->bq_b);
     D.nwave = 4 * S->C;
     return D;
 }
@@ -1544,7 +1596,7 @@ static void solver_free(nsph_solver* S) {
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->p0, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
-                      &S->flags, &S->wmax};
+                      &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
     if (S->h_pinned) (void)hipHostFree(S->h_pinned);
@@ -1797,6 +1849,10 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->aux, nf1 * 8));
     NSPH_TRY(buf_ensure(S->partial, ((size_t)S->C + 1) * 8));
     NSPH_TRY(buf_ensure(S->wmax, ((size_t)S->C + 1) * 16));
+    NSPH_TRY(buf_ensure(S->bq_count, 16));
+    NSPH_TRY(buf_ensure(S->bq_id, nf1 * 8));
+    NSPH_TRY(buf_ensure(S->bq_a, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->bq_b, nf1 * 8));
     NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->err, ((size_t)W + 1) * 8));
     NSPH_TRY(buf_ensure(S->conv, ((size_t)W + 1) * 4));
@@ -1930,8 +1986,11 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
                        Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
-    TIMED(KT_NEIGHBORS, hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st,
-                                           D, dt2));
+    TIMED(KT_NEIGHBORS, {
+        hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
+        if (S->Nb > 0)
+            hipLaunchKernelGGL(k_sph_bnd_density, dim3((unsigned)std::min(S->C, 2048)), dim3(256), 0, st, D, dt2);
+    });
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
