@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
                     [--workload dam_break_1m|dam_break_32k|batched_4096|batched_1024|xpbd_granular_1m|semi_implicit_1m]
-                    [--warm-start zero|prev] [--no-cpu-baseline]
+                    [--warm-start zero|prev] [--no-cpu-baseline] [--no-extra]
 
 Prints ONE JSON line. A step = one SolverSPH.step (hash/sort, neighbour lists, density,
 advection, IISPH pressure solve to 0.1 % mean compression, integration) over the whole
@@ -11,7 +11,12 @@ roofline: the kernel class with the largest GPU time, its algorithmic bytes per 
 (compulsory unique bytes, each array once, neighbour re-reads not counted, list bytes
 counted) divided by its mean launch time from HIP events recorded on the solver's stream
 during the timed steps. cpu_baseline: the same-order FP32 CPU restatement (oracle) of the
-same step, OpenMP over the host cores, on a bounded sample of steps.
+same step, OpenMP over all host cores, on a bounded sample of steps.
+other_configs (default run of the 1M workload, skipped with --no-extra): the other benchmark
+configurations, each with its own config and numbers -- the 32,768-particle dam break, 4096
+batched worlds, the learned warm start against the zero and previous-pressure warm starts on
+the 1M dam break at dt 1e-4 and 1e-3, and 1024 batched worlds with the learned warm start
+replayed from HIP graphs (latency percentiles).
 """
 
 from __future__ import annotations
@@ -27,6 +32,16 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# What limits each kernel class, from rocprofv3 counters on MI355X (profiles/pmc_dam_break_1m.md):
+# VALU busy (SQ_ACTIVE_INST_VALU) and L1 line accesses per cycle (TCP_TOTAL_CACHE_ACCESSES).
+LIMITERS = {
+    "neighbors_density": "valu (VALUBusy ~100 %; HBM traffic ~1 TB/s)",
+    "pressure_update": "l1 line accesses (~1 per CU cycle; TA 78 % busy, VALU 58 %)",
+    "advect": "l1 line accesses and valu (TA 76 %, VALU 77 %)",
+    "source": "l1 line accesses (TA ~76 %, VALU 64 %)",
+    "pressure_acc": "valu (90 %)",
+    "final": "l1 line accesses (VALU 56 %)",
+}
 PROBE_STEPS = 10  # eager steps with every kernel class timed, before the timed steps
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
 
@@ -82,12 +97,29 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -
     }
 
 
+def host_cpu() -> tuple[int, str]:
+    """Cores this process may run on and the CPU model name."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
 def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
     from oracle import oracle as O
 
     pb = O.problem_from_model(model, solver)
-    nthreads = os.cpu_count() or 1
-    nt = min(nthreads, 16)
+    nt, cpu_model = host_cpu()
     q, qd, pp = pb.q0, pb.qd0, None
     steps = 0
     t0 = time.perf_counter()
@@ -104,8 +136,9 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
         "unit": "particle-updates/s",
         "cores": nt,
         "kind": "port",
+        "cpu_model": cpu_model,
         "sample": f"{steps} steps of the same workload from the initial state, same-order FP32 C "
-                  f"restatement (OpenMP {nt} threads, {platform.processor() or platform.machine()})",
+                  f"restatement (OpenMP, {nt} threads on {nt} host cores, {cpu_model})",
     }
 
 
@@ -168,6 +201,129 @@ def bench_contacts(args) -> None:
         "roofline": None, "cpu_baseline": None}))
 
 
+def _timed_steps(solver, s0, s1, dt, steps: int):
+    """K synchronised steps: (ms per step, per-step latencies in ms, iterations, states)."""
+    import torch
+
+    lat, its = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ta = time.perf_counter()
+        solver.step(s0, s1, None, None, dt)
+        if not solver.report:
+            torch.cuda.synchronize()
+        lat.append((time.perf_counter() - ta) * 1e3)
+        its.append(solver.last_report["max_iterations"] if solver.last_report else None)
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, lat, its, (s0, s1)
+
+
+def _pct(lat):
+    return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))}
+
+
+def extra_configs(log) -> dict:
+    """The other benchmark configurations, each measured here with its own config."""
+    import torch
+
+    from newton_sph_dev_amd import SolverSPH, scenes
+    from newton_sph_dev_amd.learned import WarmStartMLP
+
+    data = os.path.join(os.path.dirname(os.path.abspath(__file__)), "newton_sph_dev_amd", "data")
+    out = {}
+
+    def free():
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+    # C1: 32,768-particle dam break, 200 steps from rest
+    log("C1 dam_break_32k")
+    cfg = scenes.DAM_BREAK_32K
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    sol = SolverSPH(m, cfg.support, warm_start="prev")
+    ms, lat, its, _ = _timed_steps(sol, m.state(), m.state(), scenes.DT, 200)
+    out["C1_dam_break_32k"] = {
+        "config": {"workload": cfg.name, "fluid_particles": sol.fluid_count, "steps": 200, "dt": scenes.DT,
+                   "warm_start": "prev", "from": "rest"},
+        "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
+        "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
+    del sol, m
+    free()
+
+    # C4: learned warm start on the 1M dam break against zero and previous-pressure warm starts
+    cfg = scenes.DAM_BREAK_1M
+    m = scenes.build_dam_break(cfg).finalize("cuda:0")
+    c4 = {"config": {"workload": cfg.name, "fluid_particles": None, "tolerance": 1e-3, "warmup_steps": 10,
+                     "timed_steps": 20, "from": "rest"}, "runs": {}}
+    for dt, net in ((1e-4, "warmstart_mlp.npy"), (1e-3, "warmstart_mlp_dt1e-3.npy")):
+        for mode in ("zero", "prev", "mlp"):
+            log(f"C4 dt={dt} {mode}")
+            sol = SolverSPH(m, cfg.support, warm_start="zero" if mode == "mlp" else mode)
+            if mode == "mlp":
+                sol.set_mlp(WarmStartMLP.load(os.path.join(data, net)))
+            a, b = m.state(), m.state()
+            _, _, _, (a, b) = _timed_steps(sol, a, b, dt, 10)
+            ms, lat, its, _ = _timed_steps(sol, a, b, dt, 20)
+            c4["runs"][f"dt={dt:g}/{mode}"] = {"ms_per_step": ms, "mean_iterations": float(np.mean(its)),
+                                               "network": net if mode == "mlp" else None}
+            c4["config"]["fluid_particles"] = sol.fluid_count
+            del sol, a, b
+            free()
+        r = c4["runs"]
+        k = f"dt={dt:g}"
+        c4[f"speedup_mlp_vs_zero_{k}"] = r[f"{k}/zero"]["ms_per_step"] / r[f"{k}/mlp"]["ms_per_step"]
+        c4[f"speedup_mlp_vs_prev_{k}"] = r[f"{k}/prev"]["ms_per_step"] / r[f"{k}/mlp"]["ms_per_step"]
+    out["C4_learned_warm_start_1m"] = c4
+    del m
+    free()
+
+    # C3: 4096 batched worlds (log-uniform 256..8192 fluid particles)
+    log("C3 batched_4096 (building)")
+    bcfg = scenes.BATCH_4096
+    b, H = scenes.build_batched(bcfg)
+    m = b.finalize("cuda:0")
+    del b
+    sol = SolverSPH(m, H, warm_start="prev")
+    a, c = m.state(), m.state()
+    _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 3)
+    ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 10)
+    out["C3_batched_4096"] = {
+        "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
+                   "boundary_particles": sol.boundary_count, "steps": 10, "warmup": 3, "dt": scenes.DT,
+                   "warm_start": "prev", "launch": "eager, one synchronisation per step"},
+        "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
+        "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
+    del sol, a, c, m
+    free()
+
+    # C5: 1024 batched worlds (uniform 4096..65536), learned warm start, HIP graph replays
+    log("C5 batched_1024 (building)")
+    bcfg = scenes.BATCH_1024
+    b, H = scenes.build_batched(bcfg)
+    m = b.finalize("cuda:0")
+    del b
+    sol = SolverSPH(m, H)
+    sol.set_mlp(WarmStartMLP.load(os.path.join(data, "warmstart_mlp.npy")))
+    a, c = m.state(), m.state()
+    _, _, its0, (a, c) = _timed_steps(sol, a, c, scenes.DT, 6)
+    slots = max(its0[-3:]) + 1  # the first steps from rest need more; a replay finishes eagerly if short
+    sol.use_graph, sol.graph_slots = True, slots
+    _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 2)  # captures both directions
+    ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 20)
+    out["C5_batched_1024_mlp_graph"] = {
+        "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
+                   "boundary_particles": sol.boundary_count, "steps": 20, "dt": scenes.DT, "warm_start": "mlp",
+                   "launch": f"HIP graph replay, {slots} captured Jacobi slots; one host read of the "
+                             "still-iterating count after the slots per step"},
+        "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
+        "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
+    del sol, a, c, m
+    free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +334,7 @@ def main():
     ap.add_argument("--check-every", type=int, default=1)
     ap.add_argument("--dt", type=float, default=0.0, help="time step (0: the workload's 1e-4 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other benchmark configurations")
     ap.add_argument("--graph", action="store_true", help="replay captured HIP graphs")
     ap.add_argument("--graph-slots", type=int, default=0,
                     help="Jacobi iterations captured in the graph (0: warm-up maximum + 1)")
@@ -337,6 +494,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(cfg.name, dom),
+            "traffic_source": "profiles/pmc_traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel "
+                              "(copy-kernel calibrated), committed with the kernel it measures; not measured in "
+                              "this run",
+            "limiter": LIMITERS.get(dom),
             "algorithmic_bytes_per_launch": ab[dom] * nf,
             "source": roofline_src,
         },
@@ -350,6 +511,9 @@ def main():
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(model, solver, q0, qd0, dt)
+    if not args.no_extra and cfg.name == "dam_break_1m" and args.workload == "dam_break_1m":
+        del solver, model, s0, s1
+        out["other_configs"] = extra_configs(lambda msg: print(f"[bench] {msg}", file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 
