@@ -119,6 +119,9 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
     from oracle import oracle as O
 
     pb = O.problem_from_model(model, solver)
+    # the restatement keeps no candidates between steps; it searches every step with radius H
+    # (skin 0), the cheaper of its two searches
+    pb.skin, pb.max_skin_neighbors = 0.0, 0
     nt, cpu_model = host_cpu()
     q, qd, pp = pb.q0, pb.qd0, None
     steps = 0
@@ -138,7 +141,8 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
         "kind": "port",
         "cpu_model": cpu_model,
         "sample": f"{steps} steps of the same workload from the initial state, same-order FP32 C "
-                  f"restatement (OpenMP, {nt} threads on {nt} host cores, {cpu_model})",
+                  f"restatement searching neighbours every step (OpenMP, {nt} threads on {nt} host cores, "
+                  f"{cpu_model})",
     }
 
 
@@ -332,6 +336,8 @@ def main():
     ap.add_argument("--workload", default="dam_break_1m")
     ap.add_argument("--warm-start", default="prev")
     ap.add_argument("--check-every", type=int, default=1)
+    ap.add_argument("--skin", type=float, default=None,
+                    help="relative skin of the kept neighbour candidates (default: the solver's)")
     ap.add_argument("--dt", type=float, default=0.0, help="time step (0: the workload's 1e-4 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the other benchmark configurations")
@@ -369,7 +375,7 @@ def main():
         model = scenes.build_dam_break(cfg).finalize("cuda:0")
         support = cfg.support
     solver = SolverSPH(model, support, warm_start="zero" if args.warm_start == "mlp" else args.warm_start,
-                       check_every=args.check_every)
+                       check_every=args.check_every, skin=args.skin)
     if args.warm_start == "mlp":
         from newton_sph_dev_amd.learned import WarmStartMLP
 
@@ -431,6 +437,7 @@ def main():
     N.call("nsph_solver_set_timing", solver._handle, 1 << names.index(dom))
     iters = []
     lat = []
+    searched = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -438,6 +445,7 @@ def main():
         solver.step(s0, s1, None, None, dt)  # the step report synchronises: t is the step latency
         lat.append(time.perf_counter() - t1)
         iters.append(solver.last_report["max_iterations"])
+        searched.append(solver.last_report["searched_worlds"])
         s0, s1 = s1, s0
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
@@ -484,6 +492,8 @@ def main():
             "warm_start": args.warm_start,
             "worlds": solver.W,
             "mean_iterations": float(np.mean(iters)),
+            "skin": solver.skin,
+            "searched_steps": int(np.count_nonzero(searched)),
             "mean_fluid_neighbors": nbar,
         },
         "roofline": {

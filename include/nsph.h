@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define NSPH_ABI_VERSION 2
+#define NSPH_ABI_VERSION 3
 
 enum nsph_status {
     NSPH_OK = 0,
@@ -198,6 +198,13 @@ typedef struct nsph_sph_params {
     int max_iters;
     int max_neighbors;              /* fluid-fluid list capacity per particle             */
     int warm_start;                 /* enum nsph_warm_start                               */
+    double skin;                    /* 0: lists searched every step; else in [0.01, 1]:
+                                       candidates within H (1 + skin) of each particle's
+                                       search position are kept, and a world searches
+                                       again when one of its particles is farther than
+                                       about H skin / 2 from its search position          */
+    int max_skin_neighbors;         /* capacity of the kept candidates (skin > 0), in
+                                       [max_neighbors, 8192]                              */
 } nsph_sph_params;
 
 typedef struct nsph_step_report {
@@ -206,7 +213,8 @@ typedef struct nsph_step_report {
     int sum_iterations;       /* sum over worlds                                      */
     int unconverged_worlds;   /* worlds that stopped at max_iters                     */
     int max_neighbor_count;   /* max fluid-fluid neighbours of any particle           */
-    int neighbor_overflow;    /* 1 if any list exceeded max_neighbors                 */
+    int neighbor_overflow;    /* 1 if any list (or kept-candidate set) exceeded its capacity */
+    int searched_worlds;      /* worlds whose neighbour candidates were searched anew   */
     double max_density_error; /* max over worlds of the final mean compression / rho_0 */
 } nsph_step_report;
 
@@ -227,6 +235,12 @@ typedef struct nsph_state_in {
     const float* d_p;   /* sph_pressure (N) or NULL: previous-step pressure, the input of the
                            "prev" and "mlp" warm starts (NULL: the solver's own copy of its last
                            step's pressure); entries of non-fluid particles are not read        */
+    const float* d_xb;  /* sph_search_q (N x 3) or NULL: positions around which the kept
+                           neighbour candidates were searched (NULL: the solver's own copy of
+                           its last step's; NaN entries: none).  A world whose particles all
+                           lie within the rebuild distance of them uses the candidates of
+                           these positions, else it searches around the current positions,
+                           so the step's result is a function of (q, qd, f, p, xb) alone      */
 } nsph_state_in;
 
 /* State out: */
@@ -237,6 +251,8 @@ typedef struct nsph_state_out {
                      particles and 0 elsewhere                                                 */
     float* d_p;   /* NULL, or receives the step's converged pressure (sph_pressure; 0 for
                      non-fluid particles)                                                      */
+    float* d_xb;  /* NULL, or receives the step's search positions (sph_search_q; the
+                     position of a non-fluid particle); not written when skin = 0            */
 } nsph_state_out;
 
 /* One step state_in -> state_out (SolverBase.step, solver.py:301-316).  Inactive particles:
@@ -275,6 +291,9 @@ int nsph_solver_world_stats(nsph_solver* s, int* h_iters, double* h_err, int* h_
  * d_idx[k*N + id] for k < max_neighbors, d_counts[id] (0 for non-fluid). */
 int nsph_solver_neighbors(nsph_solver* s, int* d_counts, int* d_idx);
 int nsph_solver_set_warm_start(nsph_solver* s, int mode);
+/* Search every world's candidates around the current positions at the next eager step
+ * (captured steps never force). */
+int nsph_solver_rebuild_lists(nsph_solver* s);
 /* Forget the previous-step pressure (warm start input). */
 int nsph_solver_reset_pressure(nsph_solver* s);
 /* Warm-start pressure p0 of the last step in particle order (the pressure the Jacobi

@@ -70,6 +70,8 @@ class SphParams(C.Structure):
         ("max_iters", C.c_int),
         ("max_neighbors", C.c_int),
         ("warm_start", C.c_int),
+        ("skin", C.c_double),
+        ("max_skin_neighbors", C.c_int),
     ]
 
 
@@ -81,6 +83,7 @@ class StepReport(C.Structure):
         ("unconverged_worlds", C.c_int),
         ("max_neighbor_count", C.c_int),
         ("neighbor_overflow", C.c_int),
+        ("searched_worlds", C.c_int),
         ("max_density_error", C.c_double),
     ]
 
@@ -89,15 +92,19 @@ class StepReport(C.Structure):
 
 
 class StateIn(C.Structure):
-    """nsph_state_in: device pointers of state_in's particle arrays (q, qd, f, sph pressure)."""
+    """nsph_state_in: device pointers of state_in's particle arrays (q, qd, f, sph pressure,
+    sph search positions)."""
 
-    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p)]
+    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p),
+                ("d_xb", C.c_void_p)]
 
 
 class StateOut(C.Structure):
-    """nsph_state_out: device pointers of state_out's particle arrays (q, qd, f, sph pressure)."""
+    """nsph_state_out: device pointers of state_out's particle arrays (q, qd, f, sph pressure,
+    sph search positions)."""
 
-    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p)]
+    _fields_ = [("d_q", C.c_void_p), ("d_qd", C.c_void_p), ("d_f", C.c_void_p), ("d_p", C.c_void_p),
+                ("d_xb", C.c_void_p)]
 
 
 _vp = C.c_void_p
@@ -133,6 +140,7 @@ SIGNATURES: dict[str, list] = {
     "nsph_solver_neighbors": [_vp, _vp, _vp],
     "nsph_solver_set_warm_start": [_vp, _i],
     "nsph_solver_reset_pressure": [_vp],
+    "nsph_solver_rebuild_lists": [_vp],
     "nsph_solver_num_fluid": [_vp, _pi, _pi],
     "nsph_solver_set_timing": [_vp, _i],
     "nsph_solver_set_launch": [_vp, _i],

@@ -172,9 +172,14 @@ struct RadixSort {
 };
 int radix_reserve(RadixSort& rs, int n);
 // Stable LSD radix sort of (keys, vals) over the low `bits` bits. Ping-pongs between the
-// given buffers and rs's alternates; *out_keys/*out_vals point to whichever holds the result.
+// given buffers and rs's alternates; *out_keys/*out_vals point to whichever holds the result
+// (which buffer depends on n and bits only).  guard: NULL, or a device flag; when it reads 0
+// every kernel of the sort returns at once and the buffers keep their contents.
 int radix_sort(RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits,
-               hipStream_t stream, uint32_t** out_keys, uint32_t** out_vals);
+               hipStream_t stream, uint32_t** out_keys, uint32_t** out_vals, const int* guard = nullptr);
+// The buffers radix_sort(rs, keys, vals, n, bits, ...) leaves its result in, without sorting.
+void radix_result(const RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits, uint32_t** out_keys,
+                  uint32_t** out_vals);
 void radix_free(RadixSort& rs);
 
 // ---- cell table (start,end) per key -------------------------------------------------------

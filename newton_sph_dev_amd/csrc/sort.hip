@@ -71,7 +71,8 @@ __device__ inline void bins_excl_scan(Get get, Put put, uint32_t* tmp) {
 template <int DB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict__ keys, int n,
                                                       int shift, uint32_t* __restrict__ hist,
-                                                      int n_tiles) {
+                                                      int n_tiles, const int* __restrict__ guard) {
+    if (guard && *guard == 0) return;
     constexpr int NB = 1 << DB;
     __shared__ uint32_t h[NB];
     const int t = threadIdx.x;
@@ -88,7 +89,9 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint32_t* __restrict
 }
 
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scan(uint32_t* __restrict__ hist, int n_tiles,
-                                                      uint32_t* __restrict__ digit_tot) {
+                                                      uint32_t* __restrict__ digit_tot,
+                                                      const int* __restrict__ guard) {
+    if (guard && *guard == 0) return;
     __shared__ uint32_t tmp[RS_WAVES];
     uint32_t* row = hist + (size_t)blockIdx.x * n_tiles;
     uint32_t carry = 0;
@@ -107,7 +110,9 @@ template <int DB>
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot, int n_tiles) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot, int n_tiles,
+    const int* __restrict__ guard) {
+    if (guard && *guard == 0) return;
     constexpr int NB = 1 << DB;
     constexpr uint32_t MASK = (uint32_t)(NB - 1);
     __shared__ uint32_t s_gbase[NB];   // global start of this tile's run of digit d
@@ -205,8 +210,23 @@ int radix_reserve(RadixSort& rs, int n) {
     return NSPH_OK;
 }
 
+// Passes over `bits` key bits and the digit width (9-bit digits when they save a pass).
+static int rs_passes(int bits, int* digit_bits) {
+    const int p8 = bits <= 0 ? 0 : (bits + 7) / 8, p9 = bits <= 0 ? 0 : (bits + 8) / 9;
+    *digit_bits = p9 < p8 ? 9 : 8;
+    return p9 < p8 ? p9 : p8;
+}
+
+void radix_result(const RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits, uint32_t** out_keys,
+                  uint32_t** out_vals) {
+    int db = 8;
+    const bool alt = n > 0 && (rs_passes(bits, &db) & 1) != 0;
+    *out_keys = alt ? bp<uint32_t>(rs.keys_alt) : keys;
+    *out_vals = alt ? bp<uint32_t>(rs.vals_alt) : vals;
+}
+
 int radix_sort(RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits,
-               hipStream_t stream, uint32_t** out_keys, uint32_t** out_vals) {
+               hipStream_t stream, uint32_t** out_keys, uint32_t** out_vals, const int* guard) {
     NSPH_CHECK(n <= rs.capacity, NSPH_ERR_CAPACITY, "radix sort: n=%d above capacity %d", n,
                rs.capacity);
     uint32_t *ka = keys, *va = vals, *kb = bp<uint32_t>(rs.keys_alt), *vb = bp<uint32_t>(rs.vals_alt);
@@ -216,25 +236,24 @@ int radix_sort(RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits,
         return NSPH_OK;
     }
     const int n_tiles = (n + RS_TILE - 1) / RS_TILE;
-    const int p8 = bits <= 0 ? 0 : (bits + 7) / 8, p9 = bits <= 0 ? 0 : (bits + 8) / 9;
-    const int db = p9 < p8 ? 9 : 8;  // 9-bit digits when they save a pass
-    const int passes = db == 9 ? p9 : p8;
+    int db = 8;
+    const int passes = rs_passes(bits, &db);
     for (int p = 0; p < passes; ++p) {
         const int shift = db * p;
         if (db == 9) {
             hipLaunchKernelGGL(k_rs_hist<9>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
-                               bp<uint32_t>(rs.hist), n_tiles);
+                               bp<uint32_t>(rs.hist), n_tiles, guard);
             hipLaunchKernelGGL(k_rs_scan, dim3(512), dim3(RS_BLOCK), 0, stream,
-                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot));
+                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot), guard);
             hipLaunchKernelGGL(k_rs_scatter<9>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
-                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles);
+                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles, guard);
         } else {
             hipLaunchKernelGGL(k_rs_hist<8>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
-                               bp<uint32_t>(rs.hist), n_tiles);
+                               bp<uint32_t>(rs.hist), n_tiles, guard);
             hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(RS_BLOCK), 0, stream,
-                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot));
+                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot), guard);
             hipLaunchKernelGGL(k_rs_scatter<8>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
-                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles);
+                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles, guard);
         }
         NSPH_HIP(hipGetLastError());
         uint32_t* t0 = ka; ka = kb; kb = t0;

@@ -58,6 +58,8 @@ struct PhysWorld {
     float sigma2, gsig2, gsig6, W0;  // 2 sigma, sigma/H^2, 6 sigma/H, W(0) = sigma
     float psi, eps2, visc10, rq;     // rho0*V_b, 0.01 H^2, 10 nu, padded query radius
     float g[3], rho0;
+    float Rs, Rs2, thr2;  // skin radius H (1 + skin), its square, squared rebuild displacement
+    int skin_on;          // lists are kept between steps (skin > 0)
 };
 
 // W(r) and (dW/dr)/r of the cubic spline for r^2 = r2 < H^2.
@@ -149,6 +151,18 @@ struct SphDev {
     int2* bq_id;     // queued (sorted index, world)
     float4* bq_a;    // their fluid density state: (sum, compensation, Gf.x, Gf.y)
     float2* bq_b;    //                            (Gf.z, S)
+    // lists kept between steps (skin radius Rs): per sorted particle the candidates within Rs
+    // of its search position at the world's last search (capS of them at most, 0 = no skin)
+    int capS;
+    int4* snb4;
+    int* scnt;
+    float4* xbs;     // search positions, sorted order
+    float4* xbk;     // search positions, fluid index order
+    uint32_t* okey;  // sort key of fluid index k at its world's last search
+    int* wflag;      // per world: bit 0 a particle moved too far, bit 1 new search positions
+    int* rb;         // per world: 0 kept, 1 search at the positions, 2 at state_in's search positions
+    int* anyrb;      // [0]: some world searches this step (guards the sort and reorder)
+    int* built;      // [0]: a sorted layout exists (the first step searches every world)
     int nwave;   // 4 per chunk
 };
 
@@ -299,7 +313,9 @@ __device__ inline void mark_cell_wave(const uint32_t* __restrict__ skeys, int s,
     }
 }
 
-__global__ void k_sph_reorder(const float* __restrict__ q, const float* __restrict__ qd,
+// Particles in the new sorted order, cell table, inverse permutation; with kept lists also
+// the search positions in sorted order.  Runs only when some world searches anew.
+__global__ void k_sph_reorder(SphDev D, const float* __restrict__ q, const float* __restrict__ qd,
                               const float* __restrict__ mass, const int* __restrict__ fid,
                               const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
                               const float* __restrict__ p_in,
@@ -307,7 +323,9 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
                               float4* __restrict__ vs, float* __restrict__ pws,
                               int2* __restrict__ ftable, int* __restrict__ fcend,
                               int* __restrict__ inv) {
+    if (*D.anyrb == 0) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;  // blockDim.x: a multiple of 64
+    if (s == 0) *D.built = 1;
     if (s < n) {
         const int k = (int)perm[s];
         const int id = fid[k];
@@ -315,25 +333,94 @@ __global__ void k_sph_reorder(const float* __restrict__ q, const float* __restri
         xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
         vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
         pws[s] = p_in ? p_in[id] : pprev[k];
+        if (D.capS > 0) D.xbs[s] = D.xbk[k];
     }
     mark_cell_wave(skeys, s, n, ftable, fcend);
 }
 
-// Start of a step, one thread per world and per fluid particle (fluid index k):
-//   worlds:    iteration counters and flags cleared
-//   particles: the cell of k's previous sorted key (prev_keys, zero-filled before the first
-//              build) is cleared, and k's sort key and value are written.
-// prev_keys may be the keys array itself; thread k reads entry k before writing it.
-__global__ void k_sph_begin(SphDev D, int W, int n, const uint32_t* prev_keys, int2* __restrict__ table,
-                            const float* __restrict__ q, const int* __restrict__ fid,
-                            const int* __restrict__ fworld, const GridWorld* __restrict__ gw,
-                            uint32_t* keys, uint32_t* __restrict__ vals) {
+// Particles in the sorted order of the last search (one thread per sorted fluid particle);
+// nothing to do before the first search.
+__global__ void k_sph_reorder_kept(SphDev D, const float* __restrict__ q, const float* __restrict__ qd,
+                                   const float* __restrict__ mass, const int* __restrict__ fid,
+                                   const uint32_t* __restrict__ perm, const float* __restrict__ pprev,
+                                   const float* __restrict__ p_in, int n) {
+    if (*D.built == 0) return;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = (int)perm[s];
+    const int id = fid[k];
+    D.xs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], mass[id]);
+    D.vs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.0f);
+    D.pws[s] = p_in ? p_in[id] : pprev[k];
+}
+
+// Search positions b of the step's input (one thread per fluid index k): state_in's (xb_in,
+// particle order) or, without them, those of the solver's last search (none before the first:
+// NaN).  Per world, bit 0 of wflag: some particle lies farther than the rebuild distance from
+// its b (FP32 (dx dx + dy dy) + dz dz > thr2 with d = x - b, or non-finite);  bit 1: some b
+// differs from the solver's copy, or there is no copy yet (the kept candidates are not those of b).
+__global__ void k_sph_displace(SphDev D, int n, const float* __restrict__ q, const int* __restrict__ fid,
+                               const int* __restrict__ fworld, const float* __restrict__ xb_in) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int id = fid[k];
+    const int w = fworld[k];
+    const bool built = *D.built != 0;
+    const float4 c = built ? D.xbk[k] : make_float4(NAN, NAN, NAN, 0.0f);
+    float bx = c.x, by = c.y, bz = c.z;
+    bool mis = false;
+    if (xb_in) {
+        bx = xb_in[3 * id];
+        by = xb_in[3 * id + 1];
+        bz = xb_in[3 * id + 2];
+        mis = !built || __float_as_uint(bx) != __float_as_uint(c.x) || __float_as_uint(by) != __float_as_uint(c.y) ||
+              __float_as_uint(bz) != __float_as_uint(c.z);
+    }
+    const float dx = q[3 * id] - bx, dy = q[3 * id + 1] - by, dz = q[3 * id + 2] - bz;
+    const float d2 = (dx * dx + dy * dy) + dz * dz;
+    const bool far = !(d2 <= D.phys[w].thr2);  // NaN (a non-finite position or b) counts as far
+    const int bits = (far ? 1 : 0) | (mis ? 2 : 0);
+    // one atomic per wave when the wave's particles share a world (the usual case)
+    const uint64_t act = __ballot(1);
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int w0 = __shfl(w, leader, 64);
+    if (__ballot(w != w0) == 0ull) {
+        const int any = (__ballot(bits & 1) ? 1 : 0) | (__ballot(bits & 2) ? 2 : 0);
+        if ((int)__lane_id() == leader && any) atomicOr(&D.wflag[w0], any);
+    } else if (bits) {
+        atomicOr(&D.wflag[w], bits);
+    }
+}
+
+// One thread per world: how the world's lists are made this step (rb):
+//   0  kept: the candidates of the last search are filtered again
+//   1  searched anew around the current positions (any particle farther than the rebuild
+//      distance from its search position, no skin, or forced)
+//   2  searched anew around state_in's search positions (they are not those of the kept
+//      candidates, and every particle is within the rebuild distance of them)
+// Cases 0 and 2 give the same lists: a pure function of the positions and the search positions.
+__global__ void k_sph_decide(SphDev D, int W, int force) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= W) return;
+    const int f = D.wflag[w];
+    int r = 0;
+    if (D.w_nf[w] > 0) {
+        if (!D.phys[w].skin_on || force || (f & 1)) r = 1;
+        else if (f & 2) r = 2;
+    }
+    D.rb[w] = r;
+    if (r) atomicOr(D.anyrb, 1);
+}
+
+// Start of a step, one thread per world: iteration counters and flags cleared.
+__global__ void k_sph_begin(SphDev D, int W) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < W) {
         D.iters[k] = 0;
         D.err[k] = 0.0;
         D.mixed[k] = 0;
         D.conv[k] = D.w_nf[k] == 0 ? 1 : 0;
+        D.wflag[k] = 0;
     }
     if (k == 0) {
         D.act[0] = 0;
@@ -341,13 +428,39 @@ __global__ void k_sph_begin(SphDev D, int W, int n, const uint32_t* prev_keys, i
         D.flags[0] = 0;
         D.flags[2] = 0;
         D.bq_count[0] = 0;
+        D.anyrb[0] = 0;
     }
     if (k < D.nwave) D.wmax[k] = 0;
+}
+
+// Sort keys when some world searches anew (one thread per fluid index k): the cell of k's
+// previous sorted key (prev_keys, zero-filled before the first build) is cleared; a searching
+// world's particle gets the key of its search position (the current one for rb = 1, state_in's
+// for rb = 2), which becomes its xbk; a kept world's keeps the key of its last search, so the
+// stable sort leaves that world's order as it was.  prev_keys may be the keys array itself;
+// thread k reads entry k before writing it.
+__global__ void k_sph_keys(SphDev D, int n, const uint32_t* prev_keys, int2* __restrict__ table,
+                           const float* __restrict__ q, const float* __restrict__ xb_in,
+                           const int* __restrict__ fid, const int* __restrict__ fworld,
+                           const GridWorld* __restrict__ gw, uint32_t* keys, uint32_t* __restrict__ vals) {
+    if (*D.anyrb == 0) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
         table[prev_keys[k] >> NSPH_XSUB_BITS] = make_int2(0, 0);
         const int id = fid[k];
-        const GridWorld g = gw[fworld[k]];
-        keys[k] = slab_key(g, q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+        const int w = fworld[k];
+        const int r = D.rb[w];
+        uint32_t key;
+        if (r) {
+            const float* b = r == 2 ? xb_in : q;
+            const float bx = b[3 * id], by = b[3 * id + 1], bz = b[3 * id + 2];
+            key = slab_key(gw[w], bx, by, bz);
+            if (D.capS > 0) D.xbk[k] = make_float4(bx, by, bz, 0.0f);
+        } else {
+            key = D.okey[k];
+        }
+        D.okey[k] = key;
+        keys[k] = key;
         vals[k] = (uint32_t)k;
     }
 }
@@ -688,6 +801,12 @@ __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s
 #define NBR_LDS_QUADS 6
 // Six waves per SIMD (80 VGPRs, no scratch; the LDS staging allows six workgroups of 256
 // per CU) hide more of the gathers' latency than the five the register count gives unasked.
+// SKIN = false: the candidate scan of every step gives the list (radius H).  SKIN = true:
+// candidates within the skin radius Rs are kept between steps; a world that rebuilds scans
+// them anew (sorted order of this build), and every world then filters its kept candidates to
+// the list (d^2 < H^2, same FP32 predicate, same ascending order) while adding the density
+// terms, so the list equals the one a scan with radius H in this sorted order would give.
+template <bool SKIN>
 __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -721,8 +840,8 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         }
         ++c;
     };
-    rows_visit_ids(g, D.ftable, D.fcend, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
-    {
+    // the last, partial quad, the LDS-staged quads, the count and the per-wave maximum
+    auto finish_list = [&]() {
         const int cl = c < cap ? c : cap;
         const int u = cl & 3;  // entries of the last, partial quad, at its top
         if (u == 1) put(cl >> 2, make_int4(quad.w, 0, 0, 0));
@@ -730,14 +849,12 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         if (u == 3) put(cl >> 2, make_int4(quad.y, quad.z, quad.w, 0));
         const int nq = min((cl + 3) >> 2, NBR_LDS_QUADS);
         for (int q = 0; q < nq; ++q) out[(size_t)q * Nf] = lq[q * bd + t];
-    }
-    D.ncnt[s] = c < cap ? c : cap;
-    if (c > cap) atomicOr(&D.flags[0], 1);
-    // Largest count of the step: each wave stores its own and k_sph_report takes the maximum.
-    // (A per-lane atomicMax on one address compiles to a scalar loop over the wave's lanes,
-    // about a tenth of this kernel's time; one same-address atomic per wave still costs
-    // several percent across the eight XCDs.)
-    {
+        D.ncnt[s] = cl;
+        if (c > cap) atomicOr(&D.flags[0], 1);
+        // Largest count of the step: each wave stores its own and k_sph_report takes the
+        // maximum.  (A per-lane atomicMax on one address compiles to a scalar loop over the
+        // wave's lanes, about a tenth of this kernel's time; one same-address atomic per wave
+        // still costs several percent across the eight XCDs.)
         const int wv = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
         if (__ballot(1) == ~0ull) {
             const int r = row16_max(c);
@@ -747,27 +864,73 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         } else {
             atomicMax(&D.wmax[wv], c);  // a wave with inactive lanes (the end of a world)
         }
-    }
+    };
 
     float sum = xi.w * P.W0, cmp = 0.0f;
     float gfx = 0.f, gfy = 0.f, gfz = 0.f, S = 0.f;
-    // the first NBR_LDS_QUADS quads are read back from LDS
-    for_each_nbr_q<float4, 8>(
-        c < cap ? c : cap,
-        [&](int q) { return q < NBR_LDS_QUADS ? lq[q * bd + t] : out[(size_t)q * Nf]; },
-        [&](int j) { return D.xs[j]; },
-        [&](const float4& xj, int) {
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float r2 = (dx * dx + dy * dy) + dz * dz;
-            float W, gf;
-            kern_wg(P, r2, &W, &gf);
-            neumaier(sum, cmp, xj.w * W);
-            const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
-            gfx += xj.w * gx;
-            gfy += xj.w * gy;
-            gfz += xj.w * gz;
-            S += xj.w * dot3(gx, gy, gz, gx, gy, gz);
-        });
+    auto dens = [&](float mj, float dx, float dy, float dz, float r2) {
+        float W, gf;
+        kern_wg(P, r2, &W, &gf);
+        neumaier(sum, cmp, mj * W);
+        const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+        gfx += mj * gx;
+        gfy += mj * gy;
+        gfz += mj * gz;
+        S += mj * dot3(gx, gy, gz, gx, gy, gz);
+    };
+    if constexpr (!SKIN) {
+        rows_visit_ids(g, D.ftable, D.fcend, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
+        finish_list();
+        // the first NBR_LDS_QUADS quads are read back from LDS
+        for_each_nbr_q<float4, 8>(
+            c < cap ? c : cap,
+            [&](int q) { return q < NBR_LDS_QUADS ? lq[q * bd + t] : out[(size_t)q * Nf]; },
+            [&](int j) { return D.xs[j]; },
+            [&](const float4& xj, int) {
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                dens(xj.w, dx, dy, dz, (dx * dx + dy * dy) + dz * dz);
+            });
+    } else {
+        int4* __restrict__ sk = D.snb4 + s;
+        int ns;
+        if (__builtin_amdgcn_readfirstlane(D.rb[w]) != 0) {
+            const int capS = D.capS;
+            int cs = 0;
+            int4 sq = make_int4(0, 0, 0, 0);
+            auto keep = [&](int j) {
+                if (j == s) return;
+                if (cs < capS) {
+                    sq = make_int4(sq.y, sq.z, sq.w, j);
+                    if ((cs & 3) == 3) sk[(size_t)(cs >> 2) * Nf] = sq;
+                }
+                ++cs;
+            };
+            const float4 xq = D.xbs[s];
+            rows_visit_ids(g, D.ftable, D.fcend, D.xbs, xq.x, xq.y, xq.z, P.Rs, P.Rs2, keep);
+            ns = cs < capS ? cs : capS;
+            const int u = ns & 3;
+            if (u == 1) sk[(size_t)(ns >> 2) * Nf] = make_int4(sq.w, 0, 0, 0);
+            if (u == 2) sk[(size_t)(ns >> 2) * Nf] = make_int4(sq.z, sq.w, 0, 0);
+            if (u == 3) sk[(size_t)(ns >> 2) * Nf] = make_int4(sq.y, sq.z, sq.w, 0);
+            D.scnt[s] = ns;
+            if (cs > capS) atomicOr(&D.flags[0], 1);
+        } else {
+            ns = D.scnt[s];
+        }
+        // the thread reads back the quads it stored above (same address, program order)
+        for_each_nbr_q<float4, 8>(
+            ns, [&](int q) { return sk[(size_t)q * Nf]; }, [&](int j) { return D.xs[j]; },
+            [&](const float4& xj, int j) {
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float r2 = (dx * dx + dy * dy) + dz * dz;
+                if (r2 < P.H2) {
+                    const bool within = c < cap;  // as the list: only its first cap entries count
+                    append(j);
+                    if (within) dens(xj.w, dx, dy, dz, r2);
+                }
+            });
+        finish_list();
+    }
 
     // Particles whose cell has boundary particles in its 3x3x3 block are queued with their
     // fluid sums; k_sph_bnd_density adds the boundary terms in full waves (the visit is
@@ -1061,14 +1224,15 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
 // (act_slot >= 0) the still-iterating count act[act_slot].
 struct RepDev {
     int act, max_it, sum_it, unconv, overflow, maxnbr, nonfinite, seq;  // seq: written last
+    int searched;
     double max_err;
 };
 __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_slot, int seq, RepDev* __restrict__ out) {
     constexpr int NT = 1024;
-    __shared__ int smax[NT], ssum[NT], sun[NT], snb[NT];
+    __shared__ int smax[NT], ssum[NT], sun[NT], snb[NT], ssr[NT];
     __shared__ double serr[NT];
     const int t = threadIdx.x;
-    int mx = 0, sm = 0, un = 0, nb = 0;
+    int mx = 0, sm = 0, un = 0, nb = 0, sr = 0;
     double er = 0.0;
     // per-wave list maxima, four per load (nwave = 4 per chunk), several loads in flight
     const int4* __restrict__ wm = reinterpret_cast<const int4*>(D.wmax);
@@ -1085,11 +1249,13 @@ __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_sl
         sm += it;
         un += D.conv[w] != 1 ? 1 : 0;
         er = fmax(er, D.err[w]);
+        sr += D.rb[w] != 0 ? 1 : 0;
     }
     smax[t] = mx;
     ssum[t] = sm;
     sun[t] = un;
     snb[t] = nb;
+    ssr[t] = sr;
     serr[t] = er;
     __syncthreads();
     for (int off = NT / 2; off > 0; off >>= 1) {
@@ -1098,6 +1264,7 @@ __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_sl
             ssum[t] += ssum[t + off];
             sun[t] += sun[t + off];
             snb[t] = max(snb[t], snb[t + off]);
+            ssr[t] += ssr[t + off];
             serr[t] = fmax(serr[t], serr[t + off]);
         }
         __syncthreads();
@@ -1112,6 +1279,7 @@ __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_sl
         r.maxnbr = snb[0];
         r.nonfinite = D.flags[2];
         r.seq = seq - 1;
+        r.searched = ssr[0];
         r.max_err = serr[0];
         *out = r;  // pinned host memory
         __threadfence_system();
@@ -1146,7 +1314,8 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
                                 const float* __restrict__ inv_mass, int n, float dt, float v_max,
                                 const float* __restrict__ f_in, float* __restrict__ q_out,
                                 float* __restrict__ qd_out, float* __restrict__ f_out,
-                                float* __restrict__ p_out, float* __restrict__ pprev) {
+                                float* __restrict__ p_out, float* __restrict__ xb_out,
+                                float* __restrict__ pprev) {
     if (guard && *guard != 0) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -1188,6 +1357,12 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
     }
     pprev[k] = f.w;
     if (p_out) p_out[id] = f.w;
+    if (xb_out) {
+        const float4 b = D.xbk[k];
+        xb_out[3 * id] = b.x;
+        xb_out[3 * id + 1] = b.y;
+        xb_out[3 * id + 2] = b.z;
+    }
     if (!(isfinite(x1) && isfinite(y1) && isfinite(z1) && isfinite(v1x) && isfinite(v1y) &&
           isfinite(v1z)))
         atomicOr(&D.flags[2], 1);
@@ -1295,11 +1470,16 @@ __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const floa
                                   const PhysWorld* __restrict__ phys, int W, float dt, float v_max,
                                   const float* __restrict__ f_in, float* __restrict__ q_out,
                                   float* __restrict__ qd_out, float* __restrict__ f_out,
-                                  float* __restrict__ p_out) {
+                                  float* __restrict__ p_out, float* __restrict__ xb_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int id = ids[k];
     const float x0 = q[3 * id], y0 = q[3 * id + 1], z0 = q[3 * id + 2];
+    if (xb_out) {  // not used by the solver: the particle's position
+        xb_out[3 * id] = x0;
+        xb_out[3 * id + 1] = y0;
+        xb_out[3 * id + 2] = z0;
+    }
     if (f_out) {
         f_out[3 * id] = 0.f;
         f_out[3 * id + 1] = 0.f;
@@ -1429,7 +1609,7 @@ __global__ void k_f32_zero(float* p, size_t n) {
     if (i < n) p[i] = 0.f;
 }
 
-static PhysWorld make_phys(double H, double Vb, double rho0, double nu, const float* g) {
+static PhysWorld make_phys(double H, double Vb, double rho0, double nu, const float* g, double skin) {
     PhysWorld P{};
     const double sigma = 8.0 / (M_PI * H * H * H);
     P.H = (float)H;
@@ -1448,6 +1628,15 @@ static PhysWorld make_phys(double H, double Vb, double rho0, double nu, const fl
     P.g[1] = g[1];
     P.g[2] = g[2];
     P.rho0 = (float)rho0;
+    // Kept lists: a pair within H now was within Rs at the build when each particle moved less
+    // than (Rs - H) / 2; the rebuild threshold keeps a 1e-4 relative margin against the FP32
+    // rounding of the distances and displacements.
+    const double Rs = H * (1.0 + skin);
+    const double thr = 0.5 * (Rs - H) * (1.0 - 1.0e-4);
+    P.Rs = (float)Rs;
+    P.Rs2 = (float)(Rs * Rs);
+    P.thr2 = (float)(thr * thr);
+    P.skin_on = skin > 0.0 ? 1 : 0;
     return P;
 }
 
@@ -1497,7 +1686,9 @@ struct nsph_solver {
     size_t ev_used = 0;
     double t_ms[KT_COUNT] = {0};
     long t_count[KT_COUNT] = {0};
-    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0, cap4 = 0;
+    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0, cap4 = 0, capS = 0;
+    double skin = 0.0;
+    int force_rebuild = 0;
     int gblock = CHUNK;  // gather-kernel workgroup size (64, 128 or 256)
     int key_bits = 0;
     size_t total_cells = 0;
@@ -1519,6 +1710,7 @@ struct nsph_solver {
     RadixSort rs;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, p0, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags, wmax, bq_count, bq_id, bq_a, bq_b;
+    DevBuf snb, scnt, xbs, xbk, okey, wflag, rb, anyrb, built;
     int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
     RepDev* d_rep_host = nullptr;  // device address of the pinned RepDev
     int rep_seq = 0;               // sequence number of the last step summary
@@ -1578,6 +1770,16 @@ static SphDev make_dev(nsph_solver* S) {
     D.bq_id = bp<int2>(S->bq_id);
     D.bq_a = bp<float4>(S->bq_a);
     D.bq_b = bp<float2>(S->bq_b);
+    D.capS = S->capS;
+    D.snb4 = bp<int4>(S->snb);
+    D.scnt = bp<int>(S->scnt);
+    D.xbs = bp<float4>(S->xbs);
+    D.xbk = bp<float4>(S->xbk);
+    D.okey = bp<uint32_t>(S->okey);
+    D.wflag = bp<int>(S->wflag);
+    D.rb = bp<int>(S->rb);
+    D.anyrb = bp<int>(S->anyrb);
+    D.built = bp<int>(S->built);
     D.nwave = 4 * S->C;
     return D;
 }
@@ -1596,7 +1798,9 @@ static void solver_free(nsph_solver* S) {
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
                       &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->p0, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
-                      &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b};
+                      &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b,
+                      &S->snb, &S->scnt, &S->xbs, &S->xbk, &S->okey, &S->wflag, &S->rb, &S->anyrb,
+                      &S->built};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
     if (S->h_pinned) (void)hipHostFree(S->h_pinned);
@@ -1671,6 +1875,8 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     S->warm = prm->warm_start;
     S->cap = prm->max_neighbors;
     S->cap4 = (prm->max_neighbors + 3) / 4;
+    S->skin = prm->skin;
+    S->capS = prm->skin > 0.0 ? prm->max_skin_neighbors : 0;
     S->v_max = m->v_max;
     S->d_mass = m->d_mass;
     S->d_inv_mass = m->d_inv_mass;
@@ -1727,7 +1933,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     for (int w = 0; w < W; ++w) {
         const double H = prm->h_support[w];
         NSPH_CHECK(H > 0.0 && std::isfinite(H), NSPH_ERR_INVALID, "nsph_solver_create: bad support radius in world %d", w);
-        const double cw = H * (1.0 + 1.0 / 128.0);
+        const double cw = H * (1.0 + prm->skin) * (1.0 + 1.0 / 128.0);  // cell width: the search radius
         GridWorld g{};
         for (int a = 0; a < 3; ++a) {
             const double lo = prm->h_domain_lo[3 * w + a], hi = prm->h_domain_hi[3 * w + a];
@@ -1745,7 +1951,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         NSPH_CHECK(off < (size_t)1 << 31, NSPH_ERR_INVALID, "nsph_solver_create: more than 2^31 cells in total");
         gws[w] = g;
         phys[w] = make_phys(H, prm->h_boundary_volume[w], prm->rest_density, prm->viscosity,
-                            m->h_gravity + 3 * w);
+                            m->h_gravity + 3 * w, prm->skin);
     }
     S->total_cells = off;
     NSPH_CHECK(off * NSPH_XSUB < (size_t)1 << 32, NSPH_ERR_INVALID,
@@ -1826,6 +2032,13 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     // already empty), after the boundary build has finished with the key buffer
     hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->keys), (size_t)(nmax + 1), 0);
     S->skeys = bp<uint32_t>(S->keys);
+    // The permutation is read before the sort by the kept-order copy (guarded on the device
+    // until a first sort exists), and a step captured before any eager step holds this pointer:
+    // point it at the buffer every sort of Nf keys leaves its result in.
+    {
+        uint32_t* unused = nullptr;
+        radix_result(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits, &unused, &S->perm);
+    }
     const size_t nf1 = (size_t)Nf + 1;
     NSPH_TRY(buf_ensure(S->xs, (nf1 + XS_PAD) * 16));  // windows read past a row's end
     NSPH_TRY(buf_ensure(S->vs, nf1 * 16));
@@ -1853,6 +2066,16 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->bq_id, nf1 * 8));
     NSPH_TRY(buf_ensure(S->bq_a, nf1 * 16));
     NSPH_TRY(buf_ensure(S->bq_b, nf1 * 8));
+    NSPH_TRY(buf_ensure(S->snb, nf1 * (size_t)((S->capS + 3) / 4) * 16 + 16));
+    NSPH_TRY(buf_ensure(S->scnt, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->xbs, (nf1 + XS_PAD) * 16));  // searched like xs: windows past a row's end
+    NSPH_TRY(buf_ensure(S->xbk, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->okey, nf1 * 4));
+    NSPH_TRY(buf_ensure(S->wflag, ((size_t)W + 1) * 4));
+    NSPH_TRY(buf_ensure(S->rb, ((size_t)W + 1) * 4));
+    NSPH_TRY(buf_ensure(S->anyrb, 16));
+    NSPH_TRY(buf_ensure(S->built, 16));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, bp<int>(S->built), (size_t)4, 0);
     NSPH_TRY(buf_ensure(S->iters, ((size_t)W + 1) * 4));
     NSPH_TRY(buf_ensure(S->err, ((size_t)W + 1) * 8));
     NSPH_TRY(buf_ensure(S->conv, ((size_t)W + 1) * 4));
@@ -1902,6 +2125,7 @@ static int summarise_step(nsph_solver* S, hipStream_t st, int act_slot, RepDev* 
     out->maxnbr = h->maxnbr;
     out->nonfinite = h->nonfinite;
     out->seq = h->seq;
+    out->searched = h->searched;
     out->max_err = h->max_err;
     return NSPH_OK;
 }
@@ -1914,6 +2138,7 @@ static void report_from(const RepDev& d, nsph_step_report* rep) {
     r.max_density_error = d.max_err;
     r.neighbor_overflow = d.overflow;
     r.max_neighbor_count = d.maxnbr;
+    r.searched_worlds = d.searched;
     r.status = NSPH_OK;
     if (r.unconverged_worlds) r.status = NSPH_ERR_NOT_CONVERGED;
     if (d.overflow) r.status = NSPH_ERR_CAPACITY;
@@ -1947,6 +2172,11 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* m, const nsph_sph_p
                "nsph_solver_create: rest_density, omega must be > 0 and tolerance >= 0");
     NSPH_CHECK(prm->warm_start >= NSPH_WARM_ZERO && prm->warm_start <= NSPH_WARM_MLP, NSPH_ERR_INVALID,
                "nsph_solver_create: unknown warm_start mode %d", prm->warm_start);
+    NSPH_CHECK(prm->skin == 0.0 || (prm->skin >= 0.01 && prm->skin <= 1.0), NSPH_ERR_INVALID,
+               "nsph_solver_create: skin must be 0 or in [0.01, 1] (got %g)", prm->skin);
+    NSPH_CHECK(prm->skin == 0.0 || (prm->max_skin_neighbors >= prm->max_neighbors &&
+                                    prm->max_skin_neighbors <= 8192),
+               NSPH_ERR_INVALID, "nsph_solver_create: max_skin_neighbors must be in [max_neighbors, 8192]");
     NSPH_HIP(hipSetDevice(ctx->device));
     nsph_solver* S = new nsph_solver();
     S->ctx = ctx;
@@ -1975,19 +2205,36 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     SphDev D = make_dev(S);
     EventPair* ep_build = nullptr;
     if (Nf > 0) NSPH_TRY(tic(S, KT_BUILD, &ep_build));
-    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(std::max(Nf, W), D.nwave) + 255) / 256), dim3(256), 0, st, D, W, Nf, S->skeys,
-                       bp<int2>(S->ftable), q_in, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
-                       bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(W, D.nwave) + 255) / 256), dim3(256), 0, st, D, W);
     if (Nf == 0) return NSPH_OK;
+    // With a skin, the particles are first put in the sorted order of the last search and each
+    // world's search positions are checked; the sort and reorder below then run (guarded on the
+    // device) only when some world searches anew, and leave the other worlds' order unchanged.
+    // Without a skin every world searches every step.
+    const bool keep = S->skin > 0.0;
+    if (keep) {
+        hipLaunchKernelGGL(k_sph_reorder_kept, dim3((Nf + 255) / 256), dim3(256), 0, st, D, q_in, qd_in,
+                           S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p, Nf);
+        hipLaunchKernelGGL(k_sph_displace, dim3((Nf + 255) / 256), dim3(256), 0, st, D, Nf, q_in, bp<int>(S->fid),
+                           bp<int>(S->fworld), in.d_xb);
+    }
+    hipLaunchKernelGGL(k_sph_decide, dim3((W + 255) / 256), dim3(256), 0, st, D, W, S->force_rebuild);
+    S->force_rebuild = 0;
+    hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, D, Nf, S->skeys, bp<int2>(S->ftable),
+                       q_in, in.d_xb, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
+                       bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
     NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
-                        st, &S->skeys, &S->perm));
-    hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, q_in, qd_in,
-                       S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p, S->skeys,
-                       Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
+                        st, &S->skeys, &S->perm, D.anyrb));
+    hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, D, q_in, qd_in,
+                       S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p,
+                       S->skeys, Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, {
-        hipLaunchKernelGGL(k_sph_nbr_density, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
+        if (keep)
+            hipLaunchKernelGGL(k_sph_nbr_density<true>, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
+        else
+            hipLaunchKernelGGL(k_sph_nbr_density<false>, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
         if (S->Nb > 0)
             hipLaunchKernelGGL(k_sph_bnd_density, dim3((unsigned)std::min(S->C, 2048)), dim3(256), 0, st, D, dt2);
     });
@@ -2050,14 +2297,15 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const nsph_state_in&
         TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
                                                guard, bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
                                                S->d_inv_mass, S->Nf, dt, S->v_max, in.d_f, out.d_q, out.d_qd,
-                                               out.d_f, out.d_p, bp<float>(S->pprev)));
+                                               out.d_f, out.d_p, S->capS > 0 ? out.d_xb : nullptr,
+                                               bp<float>(S->pprev)));
     }
     if (S->No > 0 && boundary) {
         TIMED(KT_PASSTHROUGH,
               hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
                                  bp<int>(S->oid), S->No, in.d_q, in.d_qd, S->d_inv_mass, S->d_flags,
                                  S->d_world, bp<PhysWorld>(S->phys), W, dt, S->v_max, in.d_f, out.d_q,
-                                 out.d_qd, out.d_f, out.d_p));
+                                 out.d_qd, out.d_f, out.d_p, S->capS > 0 ? out.d_xb : nullptr));
     }
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;
@@ -2069,7 +2317,8 @@ static int check_step_args(nsph_solver* S, const nsph_state_in* in, const nsph_s
                "nsph_solver_step: NULL state array");
     NSPH_CHECK(dt > 0.f && std::isfinite(dt), NSPH_ERR_INVALID, "nsph_solver_step: dt must be positive");
     NSPH_CHECK(S->N == 0 || (in->d_q != out->d_q && in->d_qd != out->d_qd &&
-                             (in->d_p == nullptr || in->d_p != out->d_p)),
+                             (in->d_p == nullptr || in->d_p != out->d_p) &&
+                             (in->d_xb == nullptr || in->d_xb != out->d_xb)),
                NSPH_ERR_INVALID, "nsph_solver_step: state_in and state_out must not alias");
     NSPH_CHECK(S->warm != NSPH_WARM_MLP || S->mlp_ready, NSPH_ERR_INVALID,
                "nsph_solver_step: the MLP warm start needs weights (nsph_solver_set_mlp)");
@@ -2130,6 +2379,8 @@ int nsph_solver_graph_capture(nsph_solver* S, const nsph_state_in* in, const nsp
     NSPH_HIP(hipStreamSynchronize(S->ctx->stream));
     const int timing = S->timing;
     S->timing = 0;
+    const int forced = S->force_rebuild;
+    S->force_rebuild = 0;
     StepGraph G{};
     G.in = *in;
     G.out = *out;
@@ -2173,6 +2424,7 @@ int nsph_solver_graph_capture(nsph_solver* S, const nsph_state_in* in, const nsp
         if (g) (void)hipGraphDestroy(g);
     }
     S->timing = timing;
+    S->force_rebuild = forced;
     if (rc != NSPH_OK) {
         if (G.pro) (void)hipGraphExecDestroy(G.pro);
         if (G.epi) (void)hipGraphExecDestroy(G.epi);
@@ -2278,6 +2530,12 @@ int nsph_solver_set_warm_start(nsph_solver* S, int mode) {
     NSPH_CHECK(mode >= NSPH_WARM_ZERO && mode <= NSPH_WARM_MLP, NSPH_ERR_INVALID,
                "nsph_solver_set_warm_start: unknown mode %d", mode);
     S->warm = mode;
+    return NSPH_OK;
+}
+
+int nsph_solver_rebuild_lists(nsph_solver* S) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_rebuild_lists: NULL solver");
+    S->force_rebuild = 1;
     return NSPH_OK;
 }
 

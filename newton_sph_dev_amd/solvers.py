@@ -73,6 +73,10 @@ class SolverBase:
 
 
 _WARM = {"zero": N.WARM_ZERO, "prev": N.WARM_PREV, "mlp": N.WARM_MLP}
+# skin > 0: neighbour candidates are searched within H (1 + skin) and kept between steps until
+# some particle of the world has moved more than H skin / 2 from its search position.  Off by
+# default: on MI355X the cell search costs about what filtering the kept candidates does.
+DEFAULT_SKIN = 0.0
 
 
 def _per_world(value, W: int, name: str) -> np.ndarray:
@@ -101,14 +105,26 @@ class SPHParams:
     max_iters: int
     max_neighbors: int
     warm_start: str
+    skin: float = DEFAULT_SKIN
+    max_skin_neighbors: int = 0
 
     @staticmethod
     def resolve(model: Model, smoothing_length, *, rest_density: float = 1000.0,
                 viscosity: float = 1.0e-4, boundary_volume=None, domain=None, omega: float = 0.5,
                 tolerance: float = 1.0e-3, min_iters: int = 2, max_iters: int = 100,
-                max_neighbors: int = 96, warm_start: str = "zero") -> "SPHParams":
+                max_neighbors: int = 96, warm_start: str = "zero", skin: float | None = None,
+                max_skin_neighbors: int | None = None) -> "SPHParams":
         if warm_start not in _WARM:
             raise ValueError(f"warm_start must be one of {sorted(_WARM)}")
+        skin = DEFAULT_SKIN if skin is None else float(skin)
+        if not (skin == 0.0 or 0.01 <= skin <= 1.0):
+            raise ValueError(f"skin must be 0 or in [0.01, 1] (got {skin})")
+        if max_skin_neighbors is None:
+            # the candidates within H (1 + skin) of a uniform fluid, with a 25 % margin
+            max_skin_neighbors = int(np.ceil(max_neighbors * (1.0 + skin) ** 3 * 1.25)) if skin > 0 else 0
+        max_skin_neighbors = int(max_skin_neighbors)
+        if skin > 0 and not (max_neighbors <= max_skin_neighbors <= 8192):
+            raise ValueError("max_skin_neighbors must be in [max_neighbors, 8192]")
         W = max(int(model.world_count), 1)
         h = _per_world(smoothing_length, W, "smoothing_length")
         if np.any(h <= 0):
@@ -140,7 +156,7 @@ class SPHParams:
                         hi[w] = sel.max(axis=0) + pad
         return SPHParams(W, h, vb, lo, hi, float(rest_density), float(viscosity), float(omega),
                          float(tolerance), int(min_iters), int(max_iters), int(max_neighbors),
-                         warm_start)
+                         warm_start, skin, max_skin_neighbors)
 
 
 class SolverSPH(SolverBase):
@@ -163,6 +179,15 @@ class SolverSPH(SolverBase):
         min_iters, max_iters: Jacobi iteration bounds (the step never stops silently at
             max_iters: `last_report["unconverged_worlds"]` counts such worlds).
         max_neighbors: capacity of the fluid-fluid neighbour list per particle.
+        skin: relative skin of the kept neighbour candidates, 0 (default: every step sorts
+            and searches) or in [0.01, 1]: a world's candidates are searched within
+            H (1 + skin) of each particle's search position and reused by later steps until
+            one of its particles is farther than about H skin / 2 from its own, so those steps
+            skip the sort and the cell search.  The lists are the same sets either way (in
+            the sorted order of the last search); the search positions travel with the State
+            (``sph.search_q``), so the step stays a function of the State alone.
+        max_skin_neighbors: capacity of the kept candidates per particle (default
+            max_neighbors (1 + skin)^3 1.25).
         warm_start: "zero", "prev" (0.5 * previous pressure) or "mlp".
         check_every: host convergence check interval in iterations; 0 launches max_iters
             early-exit iterations with no host read.
@@ -176,11 +201,12 @@ class SolverSPH(SolverBase):
     force, as integrate_particles integrates particle_f (solver.py:24-46); the step writes
     neither particle_f array (clear_forces() between steps, as the reference's loops do), and
     the SPH force of the last step is `sph_force` (N x 3, particle order).  With the custom
-    attribute of register_custom_attributes, each State carries the step's pressure in
-    ``state.sph.pressure`` (particle order): the step reads the warm-start input from
-    state_in and writes the converged pressure to state_out, so a recorded State resumes bit
-    for bit whatever the warm start.  Without it the solver keeps its own copy of the last
-    step's pressure.
+    attributes of register_custom_attributes, each State carries the step's pressure in
+    ``state.sph.pressure`` and the search positions of its kept neighbour candidates in
+    ``state.sph.search_q`` (particle order): the step reads the warm-start input and the
+    search positions from state_in and writes the new ones to state_out, so a recorded State
+    resumes bit for bit whatever the warm start and the skin.  Without them the solver keeps
+    its own copy of the last step's.
     """
 
     def __init__(self, model: Model, smoothing_length: float | Sequence[float], *,
@@ -189,6 +215,7 @@ class SolverSPH(SolverBase):
                  domain: tuple | None = None, omega: float = 0.5, tolerance: float = 1.0e-3,
                  min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 96,
                  warm_start: str = "zero", check_every: int = 1, report: bool = True,
+                 skin: float | None = None, max_skin_neighbors: int | None = None,
                  gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None):
         super().__init__(model)
         if model.device.type != "cuda":
@@ -197,11 +224,13 @@ class SolverSPH(SolverBase):
                                 viscosity=viscosity, boundary_volume=boundary_volume, domain=domain,
                                 omega=omega, tolerance=tolerance, min_iters=min_iters,
                                 max_iters=max_iters, max_neighbors=max_neighbors,
-                                warm_start=warm_start)
+                                warm_start=warm_start, skin=skin,
+                                max_skin_neighbors=max_skin_neighbors)
         self.params = prm
         self.W = prm.n_worlds
         for name in ("smoothing_length", "boundary_volume", "rest_density", "viscosity", "omega",
-                     "tolerance", "min_iters", "max_iters", "max_neighbors", "warm_start",
+                     "tolerance", "min_iters", "max_iters", "max_neighbors", "warm_start", "skin",
+                     "max_skin_neighbors",
                      "domain_lo", "domain_hi"):
             setattr(self, name, getattr(prm, name))
         self.check_every = int(check_every)
@@ -253,6 +282,8 @@ class SolverSPH(SolverBase):
         pr.min_iters = self.min_iters
         pr.max_iters = self.max_iters
         pr.max_neighbors = self.max_neighbors
+        pr.skin = self.skin
+        pr.max_skin_neighbors = self.max_skin_neighbors
         pr.warm_start = _WARM[self.warm_start] if self.warm_start != "mlp" else N.WARM_ZERO
         self._mlp = None
         h = C.c_void_p()
@@ -271,10 +302,16 @@ class SolverSPH(SolverBase):
     def register_custom_attributes(cls, builder: ModelBuilder) -> None:
         """Per-particle State attribute ``sph:pressure`` (solver.py:358-366 hook; custom
         attributes as builder.py:1266): the pressure of the last step, the warm-start input
-        of the next, carried in the State so that recordings and resumes keep it."""
+        of the next, carried in the State so that recordings and resumes keep it; and
+        ``sph:search_q`` (N x 3): the positions around which the step's kept neighbour
+        candidates were searched, which make the lists, like the pressure, part of the State."""
         builder.add_custom_attribute(ModelBuilder.CustomAttribute(
             name="pressure", frequency=AttributeFrequency.PARTICLE, dtype=torch.float32, default=0.0,
             assignment=AttributeAssignment.STATE, namespace="sph"))
+        # positions around which the kept neighbour candidates were searched (NaN: none yet)
+        builder.add_custom_attribute(ModelBuilder.CustomAttribute(
+            name="search_q", frequency=AttributeFrequency.PARTICLE, dtype=torch.float32,
+            default=(float("nan"),) * 3, assignment=AttributeAssignment.STATE, namespace="sph"))
 
     def set_mlp(self, mlp) -> None:
         """Attach a learned warm start (newton_sph_dev_amd.learned.WarmStartMLP) and switch
@@ -316,19 +353,26 @@ class SolverSPH(SolverBase):
         ns = getattr(state, "sph", None)
         return getattr(ns, "pressure", None) if ns is not None else None
 
+    @staticmethod
+    def _search_q_of(state: State):
+        ns = getattr(state, "sph", None)
+        return getattr(ns, "search_q", None) if ns is not None else None
+
     def _io(self, state_in: State, state_out: State) -> tuple:
         m = self.model
         for t in (state_in.particle_q, state_in.particle_qd, state_out.particle_q, state_out.particle_qd):
             if t is None or t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous():
                 raise ValueError("state arrays must be contiguous float32 tensors on the model device")
         p_in, p_out = self._pressure_of(state_in), self._pressure_of(state_out)
-        for t in (state_in.particle_f, state_out.particle_f, p_in, p_out):
+        xb_in, xb_out = self._search_q_of(state_in), self._search_q_of(state_out)
+        for t in (state_in.particle_f, state_out.particle_f, p_in, p_out, xb_in, xb_out):
             if t is not None and (t.dtype != torch.float32 or t.device != m.device or not t.is_contiguous()):
-                raise ValueError("particle_f / sph.pressure must be contiguous float32 tensors on the model device")
+                raise ValueError("particle_f / sph.pressure / sph.search_q must be contiguous float32 tensors "
+                                 "on the model device")
         si = N.StateIn(N.ptr(state_in.particle_q), N.ptr(state_in.particle_qd), N.ptr(state_in.particle_f),
-                       N.ptr(p_in))
+                       N.ptr(p_in), N.ptr(xb_in))
         so = N.StateOut(N.ptr(state_out.particle_q), N.ptr(state_out.particle_qd), N.ptr(self.sph_force),
-                        N.ptr(p_out))
+                        N.ptr(p_out), N.ptr(xb_out))
         return si, so
 
     def step(self, state_in: State, state_out: State, control: Control | None,
@@ -339,7 +383,7 @@ class SolverSPH(SolverBase):
         N.set_stream(self._ctx, torch.cuda.current_stream(m.device).cuda_stream)
         si, so = self._io(state_in, state_out)
         if self.use_graph:
-            key = (si.d_q, si.d_qd, si.d_f, si.d_p, so.d_q, so.d_qd, so.d_f, so.d_p, float(dt))
+            key = (si.d_q, si.d_qd, si.d_f, si.d_p, si.d_xb, so.d_q, so.d_qd, so.d_f, so.d_p, so.d_xb, float(dt))
             h = self._graphs.get(key)
             if h is None:
                 h = self.capture(state_in, state_out, dt, self.graph_slots)
@@ -439,6 +483,11 @@ class SolverSPH(SolverBase):
         N.call("nsph_solver_world_stats", self._handle, it.ctypes.data_as(C.POINTER(C.c_int)),
                err.ctypes.data_as(C.POINTER(C.c_double)), st.ctypes.data_as(C.POINTER(C.c_int)))
         return {"iterations": it, "density_error": err, "state": st}
+
+    def rebuild_lists(self) -> None:
+        """Search every world's neighbour candidates anew at the next step (after particles
+        were moved by other means than the step, e.g. a teleport of state_in)."""
+        N.call("nsph_solver_rebuild_lists", self._handle)
 
     def reset_pressure(self) -> None:
         """Forget the solver's own copy of the last pressure (States carrying sph.pressure

@@ -46,6 +46,8 @@ class _Problem(C.Structure):
         ("max_iters", C.c_int),
         ("max_neighbors", C.c_int),
         ("warm_start", C.c_int),
+        ("skin", C.c_double),
+        ("max_skin_neighbors", C.c_int),
     ]
 
 
@@ -92,6 +94,8 @@ class Problem:
     max_iters: int = 100
     max_neighbors: int = 64
     warm_start: int = 0
+    skin: float = 0.0
+    max_skin_neighbors: int = 0
 
     @property
     def n(self) -> int:
@@ -119,6 +123,7 @@ class Problem:
             self.rest_density, self.viscosity, self.omega, self.tolerance)
         s.min_iters, s.max_iters, s.max_neighbors, s.warm_start = (
             self.min_iters, self.max_iters, self.max_neighbors, self.warm_start)
+        s.skin, s.max_skin_neighbors = float(self.skin), int(self.max_skin_neighbors)
         return s
 
 
@@ -144,15 +149,20 @@ def problem_from_model(model, solver_like) -> Problem:
         v_max=float(model.particle_max_velocity), rest_density=solver_like.rest_density,
         viscosity=solver_like.viscosity, omega=solver_like.omega, tolerance=solver_like.tolerance,
         min_iters=solver_like.min_iters, max_iters=solver_like.max_iters,
-        max_neighbors=solver_like.max_neighbors, warm_start=warm)
+        max_neighbors=solver_like.max_neighbors, warm_start=warm,
+        skin=float(getattr(solver_like, "skin", 0.0)),
+        max_skin_neighbors=int(getattr(solver_like, "max_skin_neighbors", 0)))
 
 
 def step(pb: Problem, q_in, qd_in, pprev=None, dt: float = 1e-4, precision: str = "f64",
-         forced_iters=None, nthreads: int = 0, f_in=None) -> dict:
-    """One SPH step on the CPU. Returns q, qd, f, rho, p (particle order), neighbour counts,
-    per-world iterations and errors, the updated pprev (fluid-list order) and the status.
+         forced_iters=None, nthreads: int = 0, f_in=None, xb=None) -> dict:
+    """One SPH step on the CPU. Returns q, qd, f, rho, p, xb (particle order), neighbour
+    counts, per-world iterations and errors, the updated pprev (fluid-list order) and the status.
     f_in: optional caller force per particle (N x 3), added to the SPH force before the
-    reference integrator (solver.py:24-46 integrates the total particle_f)."""
+    reference integrator (solver.py:24-46 integrates the total particle_f).
+    xb: optional search positions of the input state (N x 3; the previous step's r["xb"]): with
+    pb.skin > 0 a world whose particles all lie within the rebuild distance of them takes its
+    neighbour candidates from them, as the solver does; None searches every world anew."""
     L = lib()
     s = pb._struct()
     n, W = pb.n, pb.n_worlds
@@ -164,15 +174,16 @@ def step(pb: Problem, q_in, qd_in, pprev=None, dt: float = 1e-4, precision: str 
     out = {
         "q": np.zeros((n, 3), np.float32), "qd": np.zeros((n, 3), np.float32),
         "f": np.zeros((n, 3), np.float32), "rho": np.zeros(n, np.float32),
-        "p": np.zeros(n, np.float32), "nbr_count": np.zeros(n, np.int32),
+        "p": np.zeros(n, np.float32), "xb": np.zeros((n, 3), np.float32), "nbr_count": np.zeros(n, np.int32),
         "iters": np.zeros(W, np.int32), "err": np.zeros(W, np.float64),
     }
     fi = None if forced_iters is None else np.ascontiguousarray(forced_iters, dtype=np.int32)
     nt = nthreads if nthreads > 0 else L.orc_num_threads()
     fn = L.orc_step_f32 if precision == "f32" else L.orc_step_f64
     fin = None if f_in is None else np.ascontiguousarray(f_in, dtype=np.float32).reshape(-1)
-    st = fn(C.byref(s), _p(q_in), _p(qd_in), _p(fin), _p(pp), C.c_float(dt), _p(fi), _p(out["q"]),
-            _p(out["qd"]), _p(out["f"]), _p(out["rho"]), _p(out["p"]), _p(out["nbr_count"]),
+    xbi = None if xb is None else np.ascontiguousarray(xb, dtype=np.float32).reshape(-1)
+    st = fn(C.byref(s), _p(q_in), _p(qd_in), _p(fin), _p(xbi), _p(pp), C.c_float(dt), _p(fi), _p(out["q"]),
+            _p(out["qd"]), _p(out["f"]), _p(out["rho"]), _p(out["p"]), _p(out["xb"]), _p(out["nbr_count"]),
             _p(out["iters"]), _p(out["err"]), C.c_int(nt))
     out["status"] = st
     out["pprev"] = pp[:nf].copy()
@@ -196,25 +207,28 @@ def neighbors_brute(x: np.ndarray, flags: np.ndarray | None, H2: float, cap: int
     return counts, idx
 
 
-def solver_neighbors(pb: Problem, q_in):
-    """Fluid-fluid lists as the solver builds them (ids, slot-major [cap, n])."""
+def solver_neighbors(pb: Problem, q_in, xb=None):
+    """Fluid-fluid lists as the solver builds them (ids, slot-major [cap, n]); xb as in step()."""
     L = lib()
     s = pb._struct()
     n = pb.n
     q_in = np.ascontiguousarray(q_in, dtype=np.float32)
+    xbi = None if xb is None else np.ascontiguousarray(xb, dtype=np.float32).reshape(-1)
     counts = np.zeros(n, np.int32)
     idx = np.full((pb.max_neighbors, n), -1, np.int32)
-    st = L.orc_solver_neighbors(C.byref(s), _p(q_in), _p(counts), _p(idx))
+    st = L.orc_solver_neighbors(C.byref(s), _p(q_in), _p(xbi), _p(counts), _p(idx))
     return counts, idx, st
 
 
-def sorted_order(pb: Problem, q_in):
+def sorted_order(pb: Problem, q_in, xb=None):
+    """Fluid ids in the solver's sorted order; xb as in step()."""
     L = lib()
     s = pb._struct()
     q_in = np.ascontiguousarray(q_in, dtype=np.float32)
+    xbi = None if xb is None else np.ascontiguousarray(xb, dtype=np.float32).reshape(-1)
     ids = np.zeros(pb.n + 1, np.int32)
     nf, nb = C.c_int(), C.c_int()
-    L.orc_sorted_order(C.byref(s), _p(q_in), _p(ids), C.byref(nf), C.byref(nb))
+    L.orc_sorted_order(C.byref(s), _p(q_in), _p(xbi), _p(ids), C.byref(nf), C.byref(nb))
     return ids[: nf.value].copy(), nf.value, nb.value
 
 

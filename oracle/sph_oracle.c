@@ -40,6 +40,8 @@ typedef struct {
     const double* domain_hi;
     double rest_density, viscosity, omega, tolerance;
     int min_iters, max_iters, max_neighbors, warm_start;
+    double skin;             /* 0: lists searched every step; > 0: kept candidates          */
+    int max_skin_neighbors;  /* capacity of the kept candidates                             */
 } orc_problem;
 
 typedef struct {
@@ -63,6 +65,7 @@ typedef struct {
     int* blist;
     int* w_fstart;
     int* w_nf;
+    float* xbk;     /* search position of each fluid-list entry (x, y, z) */
 } orc_layout;
 
 /* ---- cell geometry: identical FP32 operations to the gfx950 code --------------------- */
@@ -115,7 +118,7 @@ static size_t make_grids(const orc_problem* pb, orc_gw* gws) {
     size_t off = 0;
     for (int w = 0; w < pb->n_worlds; ++w) {
         const double H = pb->support[w];
-        const double cw = H * (1.0 + 1.0 / 128.0);
+        const double cw = H * (1.0 + pb->skin) * (1.0 + 1.0 / 128.0);  /* the search radius */
         for (int a = 0; a < 3; ++a) {
             const double lo = pb->domain_lo[3 * w + a], hi = pb->domain_hi[3 * w + a];
             double d = ceil((hi - lo) / cw) + 1.0;
@@ -130,8 +133,59 @@ static size_t make_grids(const orc_problem* pb, orc_gw* gws) {
     return off;
 }
 
-/* Hash, stably sort and tabulate; visits candidates in the same order as grid_visit. */
-static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout* L) {
+/* Squared rebuild distance and search radius of a world with support H (the same double
+ * formulas and float roundings as the solver set-up): a pair within H now was within
+ * Rs = H (1 + skin) of each other's search positions when each particle lies within
+ * (Rs - H) / 2 of its own; thr keeps a 1e-4 relative margin for the FP32 roundings. */
+static void skin_radii(double H, double skin, float* thr2, float* Rs, float* Rs2) {
+    const double rs = H * (1.0 + skin);
+    const double thr = 0.5 * (rs - H) * (1.0 - 1.0e-4);
+    *thr2 = (float)(thr * thr);
+    *Rs = (float)rs;
+    *Rs2 = (float)(rs * rs);
+}
+
+/* Candidates of the sorted points pts (x, y, z) within r of point s, in the cell visiting
+ * order of grid_visit (z, y, x over the clamped axis ranges, then sorted order inside a cell),
+ * FP32 predicate d2 < r2 with d = pts[s] - pts[j]; j == s is skipped.  Writes at most cap ids
+ * to out and returns the number found. */
+static int scan_cells(const orc_gw* g, const int* tab, const float* pts, int s, float r, float r2, int cap,
+                      int* out) {
+    const float qx = pts[3 * s], qy = pts[3 * s + 1], qz = pts[3 * s + 2];
+    const float rp = r + r * (1.0f / 256.0f);
+    int x0, nx, y0, ny, z0, nz;
+    axis_range(qx, rp, g->lo[0], g->inv_w, g->dim[0], &x0, &nx);
+    axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
+    axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
+    int c = 0;
+    for (int iz = 0; iz < nz; ++iz)
+        for (int iy = 0; iy < ny; ++iy) {
+            const int row = g->cell_off + ((z0 + iz) * g->dim[1] + (y0 + iy)) * g->dim[0];
+            for (int ix = 0; ix < nx; ++ix) {
+                const int cell = row + x0 + ix;
+                for (int j = tab[2 * cell]; j < tab[2 * cell + 1]; ++j) {
+                    if (j == s) continue;
+                    const float dx = qx - pts[3 * j], dy = qy - pts[3 * j + 1], dz = qz - pts[3 * j + 2];
+                    const float d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (!(d2 < r2)) continue;
+                    if (c < cap) out[c] = j;
+                    ++c;
+                }
+            }
+        }
+    return c;
+}
+
+/* Hash, stably sort and tabulate; visits candidates in the same order as grid_visit.
+ *
+ * Search positions (skin > 0): a world whose fluid particles all lie within the rebuild
+ * distance of their xb_in entries (FP32 (dx dx + dy dy) + dz dz <= thr2, d = x - xb; NaN
+ * fails) is sorted and searched around xb_in, any other world (and every world when xb_in is
+ * NULL or skin = 0) around the current positions.  Keys, sort and cell table are those of the
+ * search positions; the candidates of a particle are the points within Rs of its search
+ * position (search positions of both), and its list the candidates within H of it now (current
+ * positions, FP32 d2 < H^2), in candidate order, i.e. ascending sorted index. */
+static int orc_layout_build(const orc_problem* pb, const float* q_in, const float* xb_in, orc_layout* L) {
     memset(L, 0, sizeof(*L));
     const int N = pb->n, W = pb->n_worlds;
     L->cap = pb->max_neighbors;
@@ -169,17 +223,43 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
     int* ft = (int*)calloc(2 * (cells + 1), sizeof(int));
     int* bt = (int*)calloc(2 * (cells + 1), sizeof(int));
 
+    /* search positions: per world, state_in's when every particle lies within the rebuild
+     * distance of them, else the current ones */
+    const int skin_on = pb->skin > 0.0;
+    int* from_xb = (int*)calloc((size_t)W + 1, sizeof(int));
+    if (skin_on && xb_in) {
+        for (int w = 0; w < W; ++w) from_xb[w] = 1;
+        for (int k = 0; k < nf; ++k) {
+            const int id = L->fid[k], w = L->fworld[k];
+            float thr2, Rs, Rs2;
+            skin_radii(pb->support[w], pb->skin, &thr2, &Rs, &Rs2);
+            const float dx = q_in[3 * id] - xb_in[3 * id], dy = q_in[3 * id + 1] - xb_in[3 * id + 1],
+                        dz = q_in[3 * id + 2] - xb_in[3 * id + 2];
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            if (!(d2 <= thr2)) from_xb[w] = 0;
+        }
+    }
+    L->xbk = (float*)malloc(sizeof(float) * 3 * ((size_t)nf + 1));
+    for (int k = 0; k < nf; ++k) {
+        const int id = L->fid[k];
+        const float* b = from_xb[L->fworld[k]] ? xb_in : q_in;
+        L->xbk[3 * k] = b[3 * id];
+        L->xbk[3 * k + 1] = b[3 * id + 1];
+        L->xbk[3 * k + 2] = b[3 * id + 2];
+    }
+    free(from_xb);
+
     /* fluid */
     kv_t* kv = (kv_t*)malloc(sizeof(kv_t) * ((size_t)(nf > nb ? nf : nb) + 1));
     for (int k = 0; k < nf; ++k) {
-        const int id = L->fid[k];
-        kv[k].key = key_of(&gws[L->fworld[k]], q_in[3 * id], q_in[3 * id + 1], q_in[3 * id + 2]);
+        kv[k].key = key_of(&gws[L->fworld[k]], L->xbk[3 * k], L->xbk[3 * k + 1], L->xbk[3 * k + 2]);
         kv[k].idx = k;
     }
     qsort(kv, (size_t)nf, sizeof(kv_t), kv_cmp);
     L->fperm = (int*)malloc(sizeof(int) * ((size_t)nf + 1));
     L->fworld_s = (int*)malloc(sizeof(int) * ((size_t)nf + 1));
     float* fx = (float*)malloc(sizeof(float) * 3 * ((size_t)nf + 1));
+    float* fb = (float*)malloc(sizeof(float) * 3 * ((size_t)nf + 1));
     for (int s = 0; s < nf; ++s) {
         L->fperm[s] = kv[s].idx;
         L->fworld_s[s] = L->fworld[kv[s].idx];
@@ -187,6 +267,9 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
         fx[3 * s] = q_in[3 * id];
         fx[3 * s + 1] = q_in[3 * id + 1];
         fx[3 * s + 2] = q_in[3 * id + 2];
+        fb[3 * s] = L->xbk[3 * kv[s].idx];
+        fb[3 * s + 1] = L->xbk[3 * kv[s].idx + 1];
+        fb[3 * s + 2] = L->xbk[3 * kv[s].idx + 2];
         const uint32_t cell = kv[s].key >> ORC_XSUB_BITS;
         if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell] = s;
         if (s == nf - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell + 1] = s + 1;
@@ -210,60 +293,78 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
     }
     free(kv);
 
-    /* neighbour visits */
+    /* fluid lists: candidates around the search positions (radius Rs; H without a skin), then
+     * those within H of the current positions */
     const int cap = L->cap;
     L->nbr = (int*)malloc(sizeof(int) * ((size_t)nf * cap + 1));
     L->ncnt = (int*)calloc((size_t)nf + 1, sizeof(int));
+    {
+        const int capS = skin_on ? pb->max_skin_neighbors : cap;
+        int* cand = (int*)malloc(sizeof(int) * ((size_t)capS + 1));
+        for (int s = 0; s < nf; ++s) {
+            const int w = L->fworld_s[s];
+            const double H = pb->support[w];
+            const float Hf = (float)H, H2 = (float)(H * H);
+            float thr2, Rs, Rs2;
+            skin_radii(H, pb->skin, &thr2, &Rs, &Rs2);
+            int ns = skin_on ? scan_cells(&gws[w], ft, fb, s, Rs, Rs2, capS, cand)
+                             : scan_cells(&gws[w], ft, fx, s, Hf, H2, capS, cand);
+            if (ns > capS) {
+                L->overflow = 1;
+                ns = capS;
+            }
+            int c = 0;
+            for (int t = 0; t < ns; ++t) {
+                const int j = cand[t];
+                const float dx = fx[3 * s] - fx[3 * j], dy = fx[3 * s + 1] - fx[3 * j + 1],
+                            dz = fx[3 * s + 2] - fx[3 * j + 2];
+                const float d2 = (dx * dx + dy * dy) + dz * dz;
+                if (!(d2 < H2)) continue;
+                if (c < cap) L->nbr[(size_t)s * cap + c] = j;
+                ++c;
+            }
+            if (c > cap) L->overflow = 1;
+            if (c > L->max_count) L->max_count = c;
+            L->ncnt[s] = c < cap ? c : cap;
+        }
+        free(cand);
+    }
+    /* boundary visits around the current positions (radius H, same cell order) */
     L->bstart = (int*)calloc((size_t)nf + 2, sizeof(int));
     size_t bcap = (size_t)nf * 16 + 16, bn = 0;
     L->blist = (int*)malloc(sizeof(int) * bcap);
-    for (int pass = 0; pass < 2; ++pass) {
-        const int* tab = pass == 0 ? ft : bt;
-        const float* px = pass == 0 ? fx : L->bxs;
-        for (int s = 0; s < nf; ++s) {
-            const orc_gw* g = &gws[L->fworld_s[s]];
-            const double H = pb->support[L->fworld_s[s]];
-            const float Hf = (float)H, H2 = (float)(H * H);
-            const float qx = fx[3 * s], qy = fx[3 * s + 1], qz = fx[3 * s + 2];
-            const float rp = Hf + Hf * (1.0f / 256.0f);
-            int x0, nx, y0, ny, z0, nz;
-            axis_range(qx, rp, g->lo[0], g->inv_w, g->dim[0], &x0, &nx);
-            axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
-            axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
-            int c = 0;
-            if (pass == 1) L->bstart[s] = (int)bn;
-            for (int iz = 0; iz < nz; ++iz)
-                for (int iy = 0; iy < ny; ++iy) {
-                    const int row = g->cell_off + ((z0 + iz) * g->dim[1] + (y0 + iy)) * g->dim[0];
-                    for (int ix = 0; ix < nx; ++ix) {
-                        const int cell = row + x0 + ix;
-                        for (int j = tab[2 * cell]; j < tab[2 * cell + 1]; ++j) {
-                            const float dx = qx - px[3 * j], dy = qy - px[3 * j + 1], dz = qz - px[3 * j + 2];
-                            const float d2 = (dx * dx + dy * dy) + dz * dz;
-                            if (!(d2 < H2)) continue;
-                            if (pass == 0) {
-                                if (j == s) continue;
-                                if (c < cap) L->nbr[(size_t)s * cap + c] = j;
-                                ++c;
-                            } else {
-                                if (bn == bcap) {
-                                    bcap *= 2;
-                                    L->blist = (int*)realloc(L->blist, sizeof(int) * bcap);
-                                }
-                                L->blist[bn++] = j;
-                            }
+    for (int s = 0; s < nf; ++s) {
+        const orc_gw* g = &gws[L->fworld_s[s]];
+        const double H = pb->support[L->fworld_s[s]];
+        const float Hf = (float)H, H2 = (float)(H * H);
+        const float qx = fx[3 * s], qy = fx[3 * s + 1], qz = fx[3 * s + 2];
+        const float rp = Hf + Hf * (1.0f / 256.0f);
+        int x0, nx, y0, ny, z0, nz;
+        axis_range(qx, rp, g->lo[0], g->inv_w, g->dim[0], &x0, &nx);
+        axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
+        axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
+        L->bstart[s] = (int)bn;
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy) {
+                const int row = g->cell_off + ((z0 + iz) * g->dim[1] + (y0 + iy)) * g->dim[0];
+                for (int ix = 0; ix < nx; ++ix) {
+                    const int cell = row + x0 + ix;
+                    for (int j = bt[2 * cell]; j < bt[2 * cell + 1]; ++j) {
+                        const float dx = qx - L->bxs[3 * j], dy = qy - L->bxs[3 * j + 1], dz = qz - L->bxs[3 * j + 2];
+                        const float d2 = (dx * dx + dy * dy) + dz * dz;
+                        if (!(d2 < H2)) continue;
+                        if (bn == bcap) {
+                            bcap *= 2;
+                            L->blist = (int*)realloc(L->blist, sizeof(int) * bcap);
                         }
+                        L->blist[bn++] = j;
                     }
                 }
-            if (pass == 0) {
-                if (c > cap) L->overflow = 1;
-                if (c > L->max_count) L->max_count = c;
-                L->ncnt[s] = c < cap ? c : cap;
             }
-        }
-        if (pass == 1) L->bstart[nf] = (int)bn;
     }
+    L->bstart[nf] = (int)bn;
     free(fx);
+    free(fb);
     free(ft);
     free(bt);
     free(gws);
@@ -275,7 +376,7 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, orc_layout
 static void orc_layout_free(orc_layout* L) {
     free(L->fid); free(L->fworld); free(L->fperm); free(L->fworld_s); free(L->is_fluid);
     free(L->nbr); free(L->ncnt); free(L->bxs); free(L->bstart); free(L->blist);
-    free(L->w_fstart); free(L->w_nf);
+    free(L->w_fstart); free(L->w_nf); free(L->xbk);
     memset(L, 0, sizeof(*L));
 }
 
@@ -337,23 +438,25 @@ static double orc_world_reduce(const double* e, int start, int n) {
  *   f_in: NULL, or the caller's force per particle (state_in.particle_f, N x 3), added to the
  *         SPH force before integration (integrate_particles, solver.py:24-46, integrates the
  *         total force of particle_f)
+ *   xb_in: NULL, or the search positions of state_in (N x 3, see orc_layout_build)
  *   pprev: previous-step pressure in fluid-list order (ascending fluid id), in/out
  *   forced_iters: NULL, or per world the exact number of Jacobi updates (-1 = stop test)
+ *   xb_out: NULL, or receives the step's search positions (a non-fluid particle's position)
  *   returns 0, 3 (neighbour overflow) or 6 (a world hit max_iters). */
 int orc_step_f32(const orc_problem* pb, const float* q_in, const float* qd_in, const float* f_in,
-                 float* pprev, float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
-                 float* rho_out, float* p_out, int* nbr_count_out, int* iters_out,
-                 double* err_out, int nthreads) {
-    return orc_step_impl_f32(pb, q_in, qd_in, f_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
-                             rho_out, p_out, nbr_count_out, iters_out, err_out, nthreads);
+                 const float* xb_in, float* pprev, float dt, const int* forced_iters, float* q_out,
+                 float* qd_out, float* f_out, float* rho_out, float* p_out, float* xb_out,
+                 int* nbr_count_out, int* iters_out, double* err_out, int nthreads) {
+    return orc_step_impl_f32(pb, q_in, qd_in, f_in, xb_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
+                             rho_out, p_out, xb_out, nbr_count_out, iters_out, err_out, nthreads);
 }
 
 int orc_step_f64(const orc_problem* pb, const float* q_in, const float* qd_in, const float* f_in,
-                 float* pprev, float dt, const int* forced_iters, float* q_out, float* qd_out, float* f_out,
-                 float* rho_out, float* p_out, int* nbr_count_out, int* iters_out,
-                 double* err_out, int nthreads) {
-    return orc_step_impl_f64(pb, q_in, qd_in, f_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
-                             rho_out, p_out, nbr_count_out, iters_out, err_out, nthreads);
+                 const float* xb_in, float* pprev, float dt, const int* forced_iters, float* q_out,
+                 float* qd_out, float* f_out, float* rho_out, float* p_out, float* xb_out,
+                 int* nbr_count_out, int* iters_out, double* err_out, int nthreads) {
+    return orc_step_impl_f64(pb, q_in, qd_in, f_in, xb_in, pprev, dt, forced_iters, q_out, qd_out, f_out,
+                             rho_out, p_out, xb_out, nbr_count_out, iters_out, err_out, nthreads);
 }
 
 /* Cubic spline of support H at r^2 (FP64): W and (dW/dr)/r (test hook). */
@@ -407,10 +510,12 @@ int orc_neighbors_brute(const float* x, const int* flags, int n, float H2, int c
     return overflow ? 3 : 0;
 }
 
-/* Fluid particle ids in the solver's cell-sorted order, and the fluid/boundary counts. */
-int orc_sorted_order(const orc_problem* pb, const float* q_in, int* ids_out, int* nf_out, int* nb_out) {
+/* Fluid particle ids in the solver's cell-sorted order, and the fluid/boundary counts
+ * (xb_in: NULL or the search positions, as in orc_step_f32). */
+int orc_sorted_order(const orc_problem* pb, const float* q_in, const float* xb_in, int* ids_out, int* nf_out,
+                     int* nb_out) {
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, &L);
+    int rc = orc_layout_build(pb, q_in, xb_in, &L);
     if (rc) return rc;
     for (int s = 0; s < L.nf; ++s) ids_out[s] = L.fid[L.fperm[s]];
     *nf_out = L.nf;
@@ -420,10 +525,10 @@ int orc_sorted_order(const orc_problem* pb, const float* q_in, int* ids_out, int
 }
 
 /* Fluid-fluid neighbour lists as the solver builds them (particle ids, slot-major
- * idx[k * n + id] like nsph_solver_neighbors). */
-int orc_solver_neighbors(const orc_problem* pb, const float* q_in, int* counts, int* idx) {
+ * idx[k * n + id] like nsph_solver_neighbors; xb_in as in orc_step_f32). */
+int orc_solver_neighbors(const orc_problem* pb, const float* q_in, const float* xb_in, int* counts, int* idx) {
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, &L);
+    int rc = orc_layout_build(pb, q_in, xb_in, &L);
     if (rc) return rc;
     for (int i = 0; i < pb->n; ++i) counts[i] = 0;
     for (int s = 0; s < L.nf; ++s) {

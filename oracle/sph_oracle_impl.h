@@ -174,12 +174,13 @@ static void FN(pass_viscosity)(const FN(phys_t) * P, const orc_layout* Lp, const
 
 /* One step. See orc_step_* in sph_oracle.c for the argument contract. */
 static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const float* qd_in,
-                             const float* f_in, float* pprev, float dt, const int* forced_iters, float* q_out,
-                             float* qd_out, float* f_out, float* rho_out, float* p_out,
-                             int* nbr_count_out, int* iters_out, double* err_out, int nthreads) {
+                             const float* f_in, const float* xb_in, float* pprev, float dt,
+                             const int* forced_iters, float* q_out, float* qd_out, float* f_out,
+                             float* rho_out, float* p_out, float* xb_out, int* nbr_count_out,
+                             int* iters_out, double* err_out, int nthreads) {
     const int N = pb->n, W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, &L);
+    int rc = orc_layout_build(pb, q_in, xb_in, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
@@ -377,6 +378,11 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
         if (rho_out) rho_out[id] = (float)rho[s];
         if (p_out) p_out[id] = (float)p[s];
         if (nbr_count_out) nbr_count_out[id] = ncnt[s];
+        if (xb_out) {
+            xb_out[3 * id] = L.xbk[3 * k];
+            xb_out[3 * id + 1] = L.xbk[3 * k + 1];
+            xb_out[3 * id + 2] = L.xbk[3 * k + 2];
+        }
     }
     for (int s = 0; s < Nf; ++s) pprev[L.fperm[s]] = (float)p[s];
     /* non-fluid particles: reference integrator with zero force */
@@ -386,6 +392,11 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
         if (rho_out) rho_out[i] = 0.f;
         if (p_out) p_out[i] = 0.f;
         if (nbr_count_out) nbr_count_out[i] = 0;
+        if (xb_out) {  /* not used by the step: the particle's position */
+            xb_out[3 * i] = q_in[3 * i];
+            xb_out[3 * i + 1] = q_in[3 * i + 1];
+            xb_out[3 * i + 2] = q_in[3 * i + 2];
+        }
         if ((pb->flags[i] & 1) == 0) {
             q_out[3 * i] = q_in[3 * i];
             q_out[3 * i + 1] = q_in[3 * i + 1];
@@ -434,7 +445,7 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
                                  double* s_o, double* Ap_o, double* ap_o) {
     const int W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, &L);
+    int rc = orc_layout_build(pb, q_in, NULL, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
@@ -541,7 +552,7 @@ static int FN(orc_viscosity_impl)(const orc_problem* pb, const float* q_in, cons
                                   double* rho_o) {
     const int W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, &L);
+    int rc = orc_layout_build(pb, q_in, NULL, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));

@@ -209,10 +209,10 @@ def test_multistep_bitexact_and_deterministic():
         runs.append((a.particle_q.cpu().numpy(), a.particle_qd.cpu().numpy()))
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
     pb = O.problem_from_model(m, s)
-    q, qd, pp = pb.q0, pb.qd0, None
+    q, qd, pp, xb = pb.q0, pb.qd0, None, None
     for _k in range(steps):
-        r = O.step(pb, q, qd, pprev=pp, precision="f32")
-        q, qd, pp = r["q"], r["qd"], r["pprev"]
+        r = O.step(pb, q, qd, pprev=pp, precision="f32", xb=xb)
+        q, qd, pp, xb = r["q"], r["qd"], r["pprev"], r["xb"]
     assert np.array_equal(q, runs[0][0])
     assert np.array_equal(qd, runs[0][1])
 
@@ -237,10 +237,10 @@ def test_mixed_fluid_masses_bitexact():
     torch.cuda.synchronize()
     pb = O.problem_from_model(m, s)
     assert len(np.unique(pb.mass[pb.inv_mass > 0])) > 100
-    q, qd, pp = pb.q0, pb.qd0, None
+    q, qd, pp, xb = pb.q0, pb.qd0, None, None
     for _k in range(3):
-        r = O.step(pb, q, qd, pprev=pp, precision="f32")
-        q, qd, pp = r["q"], r["qd"], r["pprev"]
+        r = O.step(pb, q, qd, pprev=pp, precision="f32", xb=xb)
+        q, qd, pp, xb = r["q"], r["qd"], r["pprev"], r["xb"]
     assert np.array_equal(q, a.particle_q.cpu().numpy())
     assert np.array_equal(qd, a.particle_qd.cpu().numpy())
     assert np.array_equal(r["p"], s.fields()["pressure"].cpu().numpy())
@@ -271,12 +271,13 @@ def test_dam_break_32k_one_step():
 
 def test_list_overflow_is_reported():
     """max_neighbors below the real list lengths: the step raises CapacityError, and its
-    report carries the overflow flag and the true (uncapped) largest count."""
+    report carries the overflow flag and the true (uncapped) largest count (lists searched
+    every step, skin 0: with kept candidates the count is that of the candidates held)."""
     from newton_sph_dev_amd import _native as N
 
     cfg = scenes.small_dam_break(12, 10, 14)
     _print("dam break 12x10x14 with max_neighbors=8")
-    m, s = _solver_and_model(cfg, max_neighbors=8)
+    m, s = _solver_and_model(cfg, max_neighbors=8, skin=0.0)
     st0, st1 = m.state(), m.state()
     with pytest.raises(N.CapacityError):
         s.step(st0, st1, None, None, scenes.DT)

@@ -33,8 +33,8 @@ def test_hydrostatic_column():
     print(f"  iterations mean {np.mean(iters):.1f} max {max(iters)}; {mt}", flush=True)
     HC.check(mt)
     # the whole trajectory, step by step, equals the FP32 restatement
-    qo, qdo, pp = pb.q0, pb.qd0, None
+    qo, qdo, pp, xb = pb.q0, pb.qd0, None, None
     for _ in range(HC.STEPS):
-        r = O.step(pb, qo, qdo, pprev=pp, dt=HC.DT, precision="f32")
-        qo, qdo, pp = r["q"], r["qd"], r["pprev"]
+        r = O.step(pb, qo, qdo, pprev=pp, dt=HC.DT, precision="f32", xb=xb)
+        qo, qdo, pp, xb = r["q"], r["qd"], r["pprev"], r["xb"]
     assert np.array_equal(qo, q) and np.array_equal(qdo, qd)

@@ -40,17 +40,17 @@ def test_caller_force_is_integrated_like_the_restatement():
     a, b = m.state(), m.state()
     gen = torch.Generator(DEV).manual_seed(9)
     pb = O.problem_from_model(m, s)
-    q, qd, pp = pb.q0, pb.qd0, None
+    q, qd, pp, xb = pb.q0, pb.qd0, None, None
     for k in range(3):
         a.particle_f.normal_(0.0, 2e-3, generator=gen)  # fluid mass is 1.95e-3 kg: accelerations ~1 m/s^2
         f = a.particle_f.cpu().numpy()
         s.step(a, b, None, None, scenes.DT)
         torch.cuda.synchronize()
-        r = O.step(pb, q, qd, pprev=pp, precision="f32", f_in=f)
+        r = O.step(pb, q, qd, pprev=pp, precision="f32", f_in=f, xb=xb)
         assert np.array_equal(r["q"], b.particle_q.cpu().numpy()), k
         assert np.array_equal(r["qd"], b.particle_qd.cpu().numpy()), k
         assert np.array_equal(r["p"], b.sph.pressure.cpu().numpy()), k
-        q, qd, pp = r["q"], r["qd"], r["pprev"]
+        q, qd, pp, xb = r["q"], r["qd"], r["pprev"], r["xb"]
         a, b = b, a
     # the force changed the motion (compared with the same steps without it)
     r0 = O.step(pb, pb.q0, pb.qd0, precision="f32")

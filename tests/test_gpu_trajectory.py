@@ -72,10 +72,10 @@ def test_trajectory_drift_vs_fp64_oracle():
     torch.cuda.synchronize()
     q_gpu = a.particle_q.cpu().numpy().astype(np.float64)
     pb = O.problem_from_model(m, s)
-    q, qd, pp = pb.q0, pb.qd0, None
+    q, qd, pp, xb = pb.q0, pb.qd0, None, None
     for k in range(DRIFT_STEPS):
-        r = O.step(pb, q, qd, pprev=pp, precision="f64", forced_iters=iters[k])
-        q, qd, pp = r["q"], r["qd"], r["pprev"]
+        r = O.step(pb, q, qd, pprev=pp, precision="f64", forced_iters=iters[k], xb=xb)
+        q, qd, pp, xb = r["q"], r["qd"], r["pprev"], r["xb"]
     d = np.linalg.norm(q_gpu - q.astype(np.float64), axis=1)
     _print(f"  drift after {DRIFT_STEPS} steps: max {d.max():.3e} m, rms {np.sqrt(np.mean(d * d)):.3e} m")
     assert d.max() <= DRIFT_TOL

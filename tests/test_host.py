@@ -128,7 +128,7 @@ def test_abi_exports_every_declared_symbol():
     assert len(names) >= 25
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.nsph_abi_version() == 2
+    assert lib.nsph_abi_version() == 3
     # every declared function has a ctypes signature in the binding
     for name in names:
         assert name in N.SIGNATURES or name in ("nsph_last_error", "nsph_kernel_name"), name
