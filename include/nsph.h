@@ -304,6 +304,11 @@ int nsph_solver_num_fluid(const nsph_solver* s, int* n_fluid, int* n_boundary);
  * over 256 / gather_block workgroups). Scheduling only: results are identical for every value.
  * Set by the autotuner; captured graphs keep the value they were captured with. */
 int nsph_solver_set_launch(nsph_solver* s, int gather_block);
+/* Variant of the neighbour-search kernel: list quads per particle staged in LDS, which also
+ * sets the workgroups per CU (six of 256 up to 6 quads, five at 8).  Compiled variants: 6 (the
+ * default), 4, 8, 2; anything else fails with NSPH_ERR_INVALID.  Scheduling only: results are
+ * identical for every variant.  Set by the autotuner. */
+int nsph_solver_set_search(nsph_solver* s, int lds_quads);
 /* Per-kernel-class GPU timing with HIP events recorded on the context stream around each
  * launch group of the classes whose bit is set in `enable` (bit k = class k; -1 = all; 0 = off,
  * the default). Changing it resets the totals. nsph_solver_timing

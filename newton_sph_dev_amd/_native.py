@@ -144,6 +144,7 @@ SIGNATURES: dict[str, list] = {
     "nsph_solver_num_fluid": [_vp, _pi, _pi],
     "nsph_solver_set_timing": [_vp, _i],
     "nsph_solver_set_launch": [_vp, _i],
+    "nsph_solver_set_search": [_vp, _i],
     "nsph_solver_timing": [_vp, _pd, _pi],
     "nsph_kernel_count": [],
     "nsph_solver_iterations_launched": [_vp, _pi],

@@ -1,15 +1,19 @@
 """Autotuner for the SPH step's launch configuration.
 
 Tunables (none of them changes a single bit of the results, which the GPU tests check):
+  search         neighbour-search kernel variant: list quads per particle staged in LDS,
+                 which also sets its workgroups per CU (SolverSPH.SEARCH_VARIANTS)
   gather_block   workgroup size of the gather kernels: a 256-particle chunk is split over
-                 1, 2 or 4 workgroups of 256 / 128 / 64 threads
+                 1, 2 or 4 workgroups of 256 / 128 / 64 threads (the work partition: every
+                 world is cut into 256-particle chunks, so worlds of any size spread evenly)
   mode           "eager" with a host convergence check every `check_every` iterations, or
                  "graph": the step replayed from HIP graphs with `graph_slots` captured Jacobi
                  iterations (one host read after the slots; eager iterations finish the solve
                  if the slots were too few)
 The tuner times each candidate over a few steps of the caller's own solver and state
-(ping-pong, so the physics keeps advancing) and keeps the fastest. Results are persisted as
-JSON keyed by GPU architecture, fluid-count bucket and world count.
+(ping-pong, so the physics keeps advancing) and keeps the fastest, in two rounds: the search
+variant first (eager, the current gather block), then gather block x mode with that variant.
+Results are persisted as JSON keyed by GPU architecture, fluid-count bucket and world count.
 """
 
 from __future__ import annotations
@@ -39,19 +43,31 @@ def workload_key(solver) -> str:
     return f"{_arch()}|fluid<={bucket}|worlds={solver.W}|warm={solver.warm_start}"
 
 
-def candidates(mean_iterations: float) -> list[dict]:
+def search_candidates(gather_block: int) -> list[dict]:
+    return [{"gather_block": gather_block, "search": v, "mode": "eager", "check_every": 1}
+            for v in _variants()]
+
+
+def _variants():
+    from .solvers import SolverSPH
+
+    return SolverSPH.SEARCH_VARIANTS
+
+
+def candidates(mean_iterations: float, search=None) -> list[dict]:
     k = max(2, int(np.ceil(mean_iterations)))
+    search = search if search is not None else _variants()[0]
     out = []
     for gb in (256, 128, 64):
-        out.append({"gather_block": gb, "mode": "eager", "check_every": 1})
-        out.append({"gather_block": gb, "mode": "eager", "check_every": 2})
+        out.append({"gather_block": gb, "search": search, "mode": "eager", "check_every": 1})
+        out.append({"gather_block": gb, "search": search, "mode": "eager", "check_every": 2})
         for slots in sorted({k, k + 1}):
-            out.append({"gather_block": gb, "mode": "graph", "graph_slots": slots})
+            out.append({"gather_block": gb, "search": search, "mode": "graph", "graph_slots": slots})
     return out
 
 
 def apply(solver, cfg: dict) -> None:
-    solver.set_launch(cfg["gather_block"])
+    solver.set_launch(cfg["gather_block"], int(cfg.get("search", _variants()[0])))
     if cfg["mode"] == "graph":
         solver.use_graph = True
         solver.graph_slots = cfg["graph_slots"]
@@ -92,13 +108,22 @@ def tune(solver, state_a, state_b, dt: float, steps: int = 6, cache: Path | None
     a, b = b, a
     its = solver.last_report["max_iterations"] if solver.last_report else 2
     results = []
-    for cfg in candidates(its):
-        apply(solver, cfg)
-        _, a, b = _time(solver, a, b, dt, 1)  # warm (captures graphs)
-        ms, a, b = _time(solver, a, b, dt, steps)
-        results.append({**cfg, "ms_per_step": ms * 1e3})
-        if log:
-            log(f"autotune {cfg} -> {ms * 1e3:.3f} ms")
+
+    def run(cands):
+        nonlocal a, b
+        out = []
+        for cfg in cands:
+            apply(solver, cfg)
+            _, a, b = _time(solver, a, b, dt, 1)  # warm (captures graphs)
+            ms, a, b = _time(solver, a, b, dt, steps)
+            out.append({**cfg, "ms_per_step": ms * 1e3})
+            if log:
+                log(f"autotune {cfg} -> {ms * 1e3:.3f} ms")
+        results.extend(out)
+        return min(out, key=lambda r: r["ms_per_step"])
+
+    search = run(search_candidates(solver.gather_block))["search"]
+    run(candidates(its, search))
     best = min(results, key=lambda r: r["ms_per_step"])
     chosen = {k: v for k, v in best.items()}
     apply(solver, chosen)

@@ -798,15 +798,17 @@ __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s
     D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
 }
 
-#define NBR_LDS_QUADS 6
-// Six waves per SIMD (80 VGPRs, no scratch; the LDS staging allows six workgroups of 256
-// per CU) hide more of the gathers' latency than the five the register count gives unasked.
+// Six waves per SIMD (77-81 VGPRs, no scratch) hide more of the gathers' latency than five.
+// Variants (the launch configuration picks one; every variant gives the same bits): LQ list
+// quads per particle staged in LDS, LQ * blockDim * 16 bytes per workgroup, which also sets
+// how many workgroups of 256 share a CU (six up to LQ = 6, five at LQ = 8).  A register
+// target below six waves does not change the allocation, so occupancy is tuned through LQ.
 // SKIN = false: the candidate scan of every step gives the list (radius H).  SKIN = true:
 // candidates within the skin radius Rs are kept between steps; a world that rebuilds scans
 // them anew (sorted order of this build), and every world then filters its kept candidates to
 // the list (d^2 < H^2, same FP32 predicate, same ascending order) while adding the density
 // terms, so the list equals the one a scan with radius H in this sorted order would give.
-template <bool SKIN>
+template <bool SKIN, int LQ>
 __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_sph_nbr_density(SphDev D, float dt2) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -822,14 +824,14 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     int c = 0;
     int4 quad = make_int4(0, 0, 0, 0);
     // Entries shift into the quad from .w, so it is complete, in slot order, when every
-    // fourth entry arrives.  The first NBR_LDS_QUADS quads of a particle wait in LDS
+    // fourth entry arrives.  The first LQ quads of a particle wait in LDS
     // (lq[q * blockDim + t]) and are stored after the search, when the whole wave writes
     // quad q of its 64 particles at once; storing each quad when it completes would leave
     // lines partly written and evicted several times.
     extern __shared__ int4 lq[];
     const int t = (int)threadIdx.x, bd = (int)blockDim.x;
     auto put = [&](int q, int4 v) {
-        if (q < NBR_LDS_QUADS) lq[q * bd + t] = v;
+        if (q < LQ) lq[q * bd + t] = v;
         else out[(size_t)q * Nf] = v;
     };
     auto append = [&](int j) {
@@ -847,7 +849,7 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         if (u == 1) put(cl >> 2, make_int4(quad.w, 0, 0, 0));
         if (u == 2) put(cl >> 2, make_int4(quad.z, quad.w, 0, 0));
         if (u == 3) put(cl >> 2, make_int4(quad.y, quad.z, quad.w, 0));
-        const int nq = min((cl + 3) >> 2, NBR_LDS_QUADS);
+        const int nq = min((cl + 3) >> 2, LQ);
         for (int q = 0; q < nq; ++q) out[(size_t)q * Nf] = lq[q * bd + t];
         D.ncnt[s] = cl;
         if (c > cap) atomicOr(&D.flags[0], 1);
@@ -881,10 +883,10 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     if constexpr (!SKIN) {
         rows_visit_ids(g, D.ftable, D.fcend, D.xs, xi.x, xi.y, xi.z, P.H, P.H2, append);
         finish_list();
-        // the first NBR_LDS_QUADS quads are read back from LDS
+        // the first LQ quads are read back from LDS
         for_each_nbr_q<float4, 8>(
             c < cap ? c : cap,
-            [&](int q) { return q < NBR_LDS_QUADS ? lq[q * bd + t] : out[(size_t)q * Nf]; },
+            [&](int q) { return q < LQ ? lq[q * bd + t] : out[(size_t)q * Nf]; },
             [&](int j) { return D.xs[j]; },
             [&](const float4& xj, int) {
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1690,6 +1692,7 @@ struct nsph_solver {
     double skin = 0.0;
     int force_rebuild = 0;
     int gblock = CHUNK;  // gather-kernel workgroup size (64, 128 or 256)
+    int nbr_variant = 0;  // neighbour-kernel variant (index into kNbrQuads)
     int key_bits = 0;
     size_t total_cells = 0;
     double rho0 = 1000, omega = 0.5, tol = 1e-3;
@@ -2197,6 +2200,27 @@ int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* m, const nsph_sph_p
 // them back to back; a captured step holds prologue + a fixed number of iteration slots in
 // one graph and the epilogue in another.
 
+// Neighbour-kernel variants: LDS list quads per particle (nsph_solver_set_search).
+static const int kNbrQuads[] = {6, 4, 8, 2};
+#define NBR_VARIANTS ((int)(sizeof(kNbrQuads) / sizeof(kNbrQuads[0])))
+
+template <bool SKIN>
+static void launch_nbr_variant(int v, dim3 gg, dim3 gb, hipStream_t st, const SphDev& D, float dt2) {
+    const size_t lds = (size_t)kNbrQuads[v] * gb.x * 16;
+    switch (v) {
+        case 1: hipLaunchKernelGGL((k_sph_nbr_density<SKIN, 4>), gg, gb, lds, st, D, dt2); break;
+        case 2: hipLaunchKernelGGL((k_sph_nbr_density<SKIN, 8>), gg, gb, lds, st, D, dt2); break;
+        case 3: hipLaunchKernelGGL((k_sph_nbr_density<SKIN, 2>), gg, gb, lds, st, D, dt2); break;
+        default: hipLaunchKernelGGL((k_sph_nbr_density<SKIN, 6>), gg, gb, lds, st, D, dt2); break;
+    }
+}
+
+static void launch_nbr_density(nsph_solver* S, bool keep, dim3 gg, dim3 gb, hipStream_t st, const SphDev& D,
+                               float dt2) {
+    if (keep) launch_nbr_variant<true>(S->nbr_variant, gg, gb, st, D, dt2);
+    else launch_nbr_variant<false>(S->nbr_variant, gg, gb, st, D, dt2);
+}
+
 static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in& in, float dt) {
     const float* q_in = in.d_q;
     const float* qd_in = in.d_qd;
@@ -2231,10 +2255,7 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, {
-        if (keep)
-            hipLaunchKernelGGL(k_sph_nbr_density<true>, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
-        else
-            hipLaunchKernelGGL(k_sph_nbr_density<false>, gg, gb, (size_t)NBR_LDS_QUADS * S->gblock * 16, st, D, dt2);
+        launch_nbr_density(S, keep, gg, gb, st, D, dt2);
         if (S->Nb > 0)
             hipLaunchKernelGGL(k_sph_bnd_density, dim3((unsigned)std::min(S->C, 2048)), dim3(256), 0, st, D, dt2);
     });
@@ -2595,6 +2616,18 @@ int nsph_solver_set_launch(nsph_solver* S, int gather_block) {
                "nsph_solver_set_launch: gather_block must be 64, 128 or 256 (got %d)", gather_block);
     S->gblock = gather_block;
     return NSPH_OK;
+}
+
+int nsph_solver_set_search(nsph_solver* S, int lds_quads) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_search: NULL solver");
+    for (int v = 0; v < NBR_VARIANTS; ++v) {
+        if (kNbrQuads[v] == lds_quads) {
+            S->nbr_variant = v;
+            return NSPH_OK;
+        }
+    }
+    set_error("nsph_solver_set_search: no neighbour-kernel variant with %d LDS list quads", lds_quads);
+    return NSPH_ERR_INVALID;
 }
 
 int nsph_solver_set_timing(nsph_solver* S, int enable) {

@@ -238,6 +238,7 @@ class SolverSPH(SolverBase):
         self.shape_relaxation = 1.0
         self.last_report: dict | None = None
         self.gather_block = int(gather_block)
+        self.search_variant = self.SEARCH_VARIANTS[0]
         self.use_graph = bool(use_graph)
         self.graph_slots = graph_slots
         self._graphs: dict = {}
@@ -290,6 +291,7 @@ class SolverSPH(SolverBase):
         N.call("nsph_solver_create", self._ctx, C.byref(md), C.byref(pr), C.byref(h))
         self._handle = h
         N.call("nsph_solver_set_launch", h, self.gather_block)
+        N.call("nsph_solver_set_search", h, self.search_variant)
         # SPH force (viscosity + pressure) of the last step, particle order; the reference's
         # solvers leave state_out.particle_f to the caller, so the step does too
         self.sph_force = torch.zeros((m.particle_count, 3), dtype=torch.float32, device=m.device)
@@ -443,10 +445,17 @@ class SolverSPH(SolverBase):
         N.call("nsph_solver_graph_step", self._handle, int(handle), C.byref(rep) if rep is not None else None)
         self._check_report(rep)
 
-    def set_launch(self, gather_block: int) -> None:
-        """Gather-kernel workgroup size (64/128/256); results do not depend on it."""
+    # neighbour-search kernel variants: list quads per particle staged in LDS (the first is the default)
+    SEARCH_VARIANTS = (6, 4, 8, 2)
+
+    def set_launch(self, gather_block: int, search: int | None = None) -> None:
+        """Gather-kernel workgroup size (64/128/256) and, optionally, the neighbour-search
+        variant (LDS list quads, one of SEARCH_VARIANTS); results do not depend on either."""
         N.call("nsph_solver_set_launch", self._handle, int(gather_block))
         self.gather_block = int(gather_block)
+        if search is not None:
+            N.call("nsph_solver_set_search", self._handle, int(search))
+            self.search_variant = int(search)
         self.clear_graphs()
 
     def collect_report(self) -> dict:

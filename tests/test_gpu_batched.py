@@ -104,12 +104,15 @@ def test_graph_replay_equals_eager(slots):
 
 def test_launch_configs_do_not_change_results():
     cfg = scenes.small_batch(10, seed=5)
-    _print("gather block 64/128/256 and graph mode give identical steps")
+    _print("gather block 64/128/256, every search variant and graph mode give identical steps")
     b, H = scenes.build_batched(cfg)
     m = b.finalize("cuda:0")
     outs = []
-    for gb, graph in ((256, False), (128, False), (64, False), (128, True)):
+    configs = [(256, False, 6), (128, False, 6), (64, False, 8), (128, True, 4)]
+    configs += [(256, False, v) for v in SolverSPH.SEARCH_VARIANTS[1:]]
+    for gb, graph, search in configs:
         s = SolverSPH(m, H, warm_start="prev", gather_block=gb, use_graph=graph, graph_slots=2)
+        s.set_launch(gb, search)
         a, c = m.state(), m.state()
         for _ in range(4):
             s.step(a, c, None, None, scenes.DT)
@@ -156,6 +159,8 @@ def test_autotune_picks_a_config(tmp_path):
     a, c = m.state(), m.state()
     best = autotune.tune(s, a, c, scenes.DT, steps=2, cache=tmp_path / "tune.json")
     assert best["gather_block"] in (64, 128, 256) and best["mode"] in ("eager", "graph")
+    assert best["search"] in SolverSPH.SEARCH_VARIANTS and s.search_variant == best["search"]
+    assert len({r["search"] for r in best["candidates"]}) == len(SolverSPH.SEARCH_VARIANTS)
     assert (tmp_path / "tune.json").exists()
     again = autotune.tune(s, a, c, scenes.DT, cache=tmp_path / "tune.json")
     assert again["gather_block"] == best["gather_block"]
