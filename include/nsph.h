@@ -219,9 +219,11 @@ typedef struct nsph_step_report {
 } nsph_step_report;
 
 /* Create the solver: classifies particles (fluid = ACTIVE and inv_mass > 0; static
- * boundary = ACTIVE, inv_mass == 0 and zero velocity; the rest inactive) and hashes the
- * static boundary once. Fails with NSPH_ERR_INVALID for a zero-mass ACTIVE particle with
- * non-zero velocity (moving boundaries are not supported). */
+ * boundary = ACTIVE, inv_mass == 0 and zero velocity in the model; moving boundary = ACTIVE,
+ * inv_mass == 0 and a velocity in the model; the rest inactive) and hashes the static
+ * boundary once.  Moving boundary particles are integrated like every zero-mass particle
+ * (x + v dt, solver.py:24-46) and hashed from state_in at each step, where their velocity
+ * enters the source term as psi sum_b (v_i - v_b) . gradW_ib. */
 int nsph_solver_create(nsph_ctx* ctx, const nsph_model_desc* model, const nsph_sph_params* params,
                        nsph_solver** out);
 /* The particle arrays of one State (newton/_src/sim/state.py:101-108), particle order, device

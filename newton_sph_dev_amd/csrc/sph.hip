@@ -117,6 +117,14 @@ struct SphDev {
     const int* bcend;
     const float4* bxs;
     const unsigned char* breach;  // per cell: 1 if a boundary particle lies in its 3x3x3 block
+    // moving boundary (zero-mass particles with a velocity), hashed every step from state_in;
+    // kxs == NULL when there is none
+    const int2* ktable;
+    const int* kcend;
+    const float4* kxs;
+    const float4* kvs;
+    const unsigned char* kbreach;  // per cell: 1 if a moving boundary particle lies in its 3x3x3 block
+    float* kvb;  // per sorted fluid particle: sum_b v_b . gradW_ib over moving boundary particles
     // fluid grid
     int2* ftable;
     int* fcend;
@@ -787,7 +795,7 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 // G = Gf + psi Gb, a_ii = -dt^2 ((G.G + m_i S) / rho^2), and the per-particle outputs.
 __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s, float mi, float sum, float cmp,
                                       float gfx, float gfy, float gfz, float S, float gbx, float gby, float gbz,
-                                      float mr2, int nb, float dt2) {
+                                      float mr2, int nb, float dt2, float kvb = 0.f) {
     const float rho = sum + cmp;
     const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
     const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
@@ -796,6 +804,7 @@ __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s
     D.aux[s] = make_float2((float)nb, nb > 0 ? sqrtf(mr2) * P.invH : 1.0f);
     D.aii[s] = -(dt2 * (num / (rho * rho)));
     D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
+    if (D.kvb) D.kvb[s] = kvb;
 }
 
 // Six waves per SIMD (77-81 VGPRs, no scratch) hide more of the gathers' latency than five.
@@ -937,7 +946,8 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     // Particles whose cell has boundary particles in its 3x3x3 block are queued with their
     // fluid sums; k_sph_bnd_density adds the boundary terms in full waves (the visit is
     // per-lane and a wave holds few such particles).  The others finish here.
-    const bool breach = D.breach[g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx] != 0;
+    const int cell = g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx;
+    const bool breach = D.breach[cell] != 0 || (D.kbreach != nullptr && D.kbreach[cell] != 0);
     const uint64_t bm = __ballot(breach);
     if (bm) {
         const int lane = (int)__lane_id();
@@ -983,7 +993,24 @@ __global__ __launch_bounds__(256) void k_sph_bnd_density(SphDev D, float dt2) {
                        mr2 = fminf(mr2, r2);
                        ++nb;
                    });
-        density_finish(D, P, s, xi.w, sum, cmp, a.z, a.w, b.x, b.y, gbx, gby, gbz, mr2, nb, dt2);
+        // moving boundary particles after the static ones, same terms, plus v_b . gradW_ib
+        float kvb = 0.f;
+        if (D.kxs != nullptr)
+            rows_visit(g, D.ktable, D.kcend, D.kxs, xi.x, xi.y, xi.z, P.H, P.H2,
+                       [&](int j, float dx, float dy, float dz, float r2) {
+                           float W, gf;
+                           kern_wg(P, r2, &W, &gf);
+                           neumaier(sum, cmp, P.psi * W);
+                           const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+                           gbx += gx;
+                           gby += gy;
+                           gbz += gz;
+                           const float4 vb = D.kvs[j];
+                           kvb += dot3(vb.x, vb.y, vb.z, gx, gy, gz);
+                           mr2 = fminf(mr2, r2);
+                           ++nb;
+                       });
+        density_finish(D, P, s, xi.w, sum, cmp, a.z, a.w, b.x, b.y, gbx, gby, gbz, mr2, nb, dt2, kvb);
     }
 }
 
@@ -1062,7 +1089,8 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
             az += c * (gf * dz);
         });
     const float4 gb = D.gb[s];
-    sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
+    // boundary: psi sum_b (v_i - v_b) . gradW_ib (v_b = 0 for static boundary particles)
+    sum += P.psi * (dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z) - (D.kvb ? D.kvb[s] : 0.f));
     D.src[s] = (P.rho0 - rho) - dt * sum;
     D.p[s] = p0;
     D.p0[s] = p0;
@@ -1087,7 +1115,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int wa
             sum += o.x.w * dot3(vi.x - o.a.x, vi.y - o.a.y, vi.z - o.a.z, gf * dx, gf * dy, gf * dz);
         });
     const float4 gb = D.gb[s];
-    sum += P.psi * dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z);
+    sum += P.psi * (dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z) - (D.kvb ? D.kvb[s] : 0.f));
     const float rho = D.rho[s];
     D.src[s] = (P.rho0 - rho) - dt * sum;
     float p0 = 0.f;
@@ -1576,6 +1604,56 @@ __global__ void k_bnd_reach(const GridWorld* __restrict__ gw, int W, int n_cells
     breach[c] = any;
 }
 
+// Moving boundary (zero-mass particles with a velocity), every step, one thread per moving
+// particle k: the cell of the previous step's key of k is cleared (prev_keys, zero-filled before
+// the first step), then k's key from its position in state_in.  prev_keys may be the keys array
+// itself; thread k reads entry k before writing it.
+__global__ void k_kin_keys(int n, const uint32_t* prev_keys, int2* __restrict__ table, const float* __restrict__ q,
+                           const int* __restrict__ kid, const int* __restrict__ kworld,
+                           const GridWorld* __restrict__ gw, uint32_t* keys, uint32_t* __restrict__ vals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    table[prev_keys[k] >> NSPH_XSUB_BITS] = make_int2(0, 0);
+    const int id = kid[k];
+    keys[k] = slab_key(gw[kworld[k]], q[3 * id], q[3 * id + 1], q[3 * id + 2]);
+    vals[k] = (uint32_t)k;
+}
+
+// Moving boundary in sorted order (positions and velocities of state_in), its cell table, and
+// kbreach (zeroed before) over the 3x3x3 block around each occupied cell, marked by the cell's
+// first particle.
+__global__ void k_kin_reorder(const float* __restrict__ q, const float* __restrict__ qd, const int* __restrict__ kid,
+                              const int* __restrict__ kworld, const GridWorld* __restrict__ gw,
+                              const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys, int n,
+                              float4* __restrict__ kxs, float4* __restrict__ kvs, int2* __restrict__ table,
+                              int* __restrict__ cend, unsigned char* __restrict__ kbreach) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int k = (int)perm[s];
+    const int id = kid[k];
+    kxs[s] = make_float4(q[3 * id], q[3 * id + 1], q[3 * id + 2], 0.f);
+    kvs[s] = make_float4(qd[3 * id], qd[3 * id + 1], qd[3 * id + 2], 0.f);
+    mark_cell(skeys, s, n, table, cend);
+    const int cell = (int)(skeys[s] >> NSPH_XSUB_BITS);
+    if (s > 0 && (int)(skeys[s - 1] >> NSPH_XSUB_BITS) == cell) return;
+    const GridWorld g = gw[kworld[k]];
+    const int local = cell - g.cell_off;
+    const int cx = local % g.dim[0], cy = (local / g.dim[0]) % g.dim[1], cz = local / (g.dim[0] * g.dim[1]);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dim[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= g.dim[1]) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = cx + dx;
+                if (x < 0 || x >= g.dim[0]) continue;
+                kbreach[g.cell_off + (z * g.dim[1] + y) * g.dim[0] + x] = 1;
+            }
+        }
+    }
+}
+
 // Sorted -> particle order for the debugging / dump entry points.
 __global__ void k_sph_fields(const uint32_t* __restrict__ perm, const int* __restrict__ fid, int n,
                              const float* __restrict__ rho, const float* __restrict__ p,
@@ -1688,7 +1766,7 @@ struct nsph_solver {
     size_t ev_used = 0;
     double t_ms[KT_COUNT] = {0};
     long t_count[KT_COUNT] = {0};
-    int N = 0, W = 0, Nf = 0, Nb = 0, No = 0, C = 0, cap = 0, cap4 = 0, capS = 0;
+    int N = 0, W = 0, Nf = 0, Nb = 0, Nk = 0, No = 0, C = 0, cap = 0, cap4 = 0, capS = 0;
     double skin = 0.0;
     int force_rebuild = 0;
     int gblock = CHUNK;  // gather-kernel workgroup size (64, 128 or 256)
@@ -1711,6 +1789,12 @@ struct nsph_solver {
     uint32_t* skeys = nullptr;
     uint32_t* perm = nullptr;
     RadixSort rs;
+    // moving boundary particles: ids, worlds, keys (kskeys/kperm: sorted output), sorted
+    // positions and velocities, cell table, reach bytes, per-fluid v_b . gradW sums
+    DevBuf kid, kworld, kkeys, kvals, kxs, kvs, ktable, kcend, kbreach, kvb;
+    uint32_t* kskeys = nullptr;
+    uint32_t* kperm = nullptr;
+    RadixSort rsk;
     DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, p0, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags, wmax, bq_count, bq_id, bq_a, bq_b;
     DevBuf snb, scnt, xbs, xbk, okey, wflag, rb, anyrb, built;
@@ -1743,6 +1827,13 @@ static SphDev make_dev(nsph_solver* S) {
     D.fcend = bp<int>(S->fcend);
     D.bxs = bp<float4>(S->bxs);
     D.breach = bp<unsigned char>(S->breach);
+    const bool moving = S->Nk > 0;
+    D.ktable = moving ? bp<int2>(S->ktable) : nullptr;
+    D.kcend = moving ? bp<int>(S->kcend) : nullptr;
+    D.kxs = moving ? bp<float4>(S->kxs) : nullptr;
+    D.kvs = moving ? bp<float4>(S->kvs) : nullptr;
+    D.kbreach = moving ? bp<unsigned char>(S->kbreach) : nullptr;
+    D.kvb = moving ? bp<float>(S->kvb) : nullptr;
     D.ftable = bp<int2>(S->ftable);
     D.xs = bp<float4>(S->xs);
     D.vs = bp<float4>(S->vs);
@@ -1803,9 +1894,11 @@ static void solver_free(nsph_solver* S) {
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b,
                       &S->snb, &S->scnt, &S->xbs, &S->xbk, &S->okey, &S->wflag, &S->rb, &S->anyrb,
-                      &S->built};
+                      &S->built, &S->kid, &S->kworld, &S->kkeys, &S->kvals, &S->kxs, &S->kvs, &S->ktable,
+                      &S->kcend, &S->kbreach, &S->kvb};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
+    radix_free(S->rsk);
     if (S->h_pinned) (void)hipHostFree(S->h_pinned);
     S->h_pinned = nullptr;
     for (StepGraph& g : S->graphs) {
@@ -1898,7 +1991,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
             NSPH_HIP(hipMemcpyAsync(world.data(), m->d_world, (size_t)N * 4, hipMemcpyDeviceToHost, st));
         NSPH_HIP(hipStreamSynchronize(st));
     }
-    std::vector<int> fid, fworld, bid, bworld, oid;
+    std::vector<int> fid, fworld, bid, bworld, oid, kid, kworld;
     std::vector<int> nf(W, 0), nb(W, 0);
     for (int i = 0; i < N; ++i) {
         int w = world[i] < 0 ? 0 : world[i];
@@ -1912,13 +2005,18 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         } else {
             oid.push_back(i);
             if (active) {
-                NSPH_CHECK(qd[3 * (size_t)i] == 0.f && qd[3 * (size_t)i + 1] == 0.f && qd[3 * (size_t)i + 2] == 0.f,
-                           NSPH_ERR_INVALID,
-                           "nsph_solver_create: zero-mass particle %d has non-zero velocity; "
-                           "moving boundary particles are not supported", i);
-                bid.push_back(i);
-                bworld.push_back(w);
-                nb[w]++;
+                // zero-mass: static boundary at rest, moving boundary with a velocity (the
+                // integrator moves it with that velocity, solver.py:24-46)
+                const bool rest = qd[3 * (size_t)i] == 0.f && qd[3 * (size_t)i + 1] == 0.f &&
+                                  qd[3 * (size_t)i + 2] == 0.f;
+                if (rest) {
+                    bid.push_back(i);
+                    bworld.push_back(w);
+                    nb[w]++;
+                } else {
+                    kid.push_back(i);
+                    kworld.push_back(w);
+                }
             }
         }
     }
@@ -1926,6 +2024,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     // by key groups worlds.  Fluid ids are ascending; the stable sort keeps id order per cell.
     S->Nf = (int)fid.size();
     S->Nb = (int)bid.size();
+    S->Nk = (int)kid.size();
     S->No = (int)oid.size();
     S->h_nf = nf;
 
@@ -2023,6 +2122,28 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         buf_free(dbid);
         buf_free(dbworld);
         NSPH_TRY(rc);
+        NSPH_HIP(hipGetLastError());
+    }
+
+    // moving boundary: tables zeroed, keys zero-filled for the first step's clearing
+    if (S->Nk > 0) {
+        const int Nk = S->Nk;
+        NSPH_TRY(upload(S->kid, kid, st));
+        NSPH_TRY(upload(S->kworld, kworld, st));
+        NSPH_TRY(radix_reserve(S->rsk, Nk));
+        NSPH_TRY(buf_ensure(S->kkeys, (size_t)(Nk + 1) * 4));
+        NSPH_TRY(buf_ensure(S->kvals, (size_t)(Nk + 1) * 4));
+        NSPH_TRY(buf_ensure(S->kxs, (size_t)(Nk + 1) * 16));
+        NSPH_TRY(buf_ensure(S->kvs, (size_t)(Nk + 1) * 16));
+        NSPH_TRY(buf_ensure(S->ktable, (off + 1) * sizeof(int2)));
+        NSPH_TRY(buf_ensure(S->kcend, (off + 1) * sizeof(int)));
+        NSPH_TRY(buf_ensure(S->kbreach, (off + 4) & ~(size_t)3));
+        NSPH_TRY(buf_ensure(S->kvb, ((size_t)Nf + 1) * 4));
+        hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ktable), off + 1, make_int2(0, 0));
+        hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->kkeys), (size_t)(Nk + 1), 0);
+        S->kskeys = bp<uint32_t>(S->kkeys);
+        uint32_t* unused = nullptr;
+        radix_result(S->rsk, bp<uint32_t>(S->kkeys), bp<uint32_t>(S->kvals), Nk, S->key_bits, &unused, &S->kperm);
         NSPH_HIP(hipGetLastError());
     }
 
@@ -2252,6 +2373,21 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, D, q_in, qd_in,
                        S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p,
                        S->skeys, Nf, D.xs, D.vs, D.pws, D.ftable, D.fcend, bp<int>(S->inv));
+    if (S->Nk > 0) {
+        // moving boundary: hashed from state_in every step
+        const int Nk = S->Nk;
+        const size_t words = (S->total_cells + 4) / 4;
+        hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<size_t>((words + 255) / 256, 4096)), dim3(256), 0, st,
+                           bp<int>(S->kbreach), words, 0);
+        hipLaunchKernelGGL(k_kin_keys, dim3((Nk + 255) / 256), dim3(256), 0, st, Nk, S->kskeys, bp<int2>(S->ktable),
+                           q_in, bp<int>(S->kid), bp<int>(S->kworld), bp<GridWorld>(S->gw), bp<uint32_t>(S->kkeys),
+                           bp<uint32_t>(S->kvals));
+        NSPH_TRY(radix_sort(S->rsk, bp<uint32_t>(S->kkeys), bp<uint32_t>(S->kvals), Nk, S->key_bits, st, &S->kskeys,
+                            &S->kperm));
+        hipLaunchKernelGGL(k_kin_reorder, dim3((Nk + 255) / 256), dim3(256), 0, st, q_in, qd_in, bp<int>(S->kid),
+                           bp<int>(S->kworld), bp<GridWorld>(S->gw), S->kperm, S->kskeys, Nk, bp<float4>(S->kxs),
+                           bp<float4>(S->kvs), bp<int2>(S->ktable), bp<int>(S->kcend), bp<unsigned char>(S->kbreach));
+    }
     NSPH_TRY(toc(S, ep_build));
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, {
@@ -2587,7 +2723,7 @@ int nsph_solver_warm_pressure(nsph_solver* S, float* d_p0) {
 int nsph_solver_num_fluid(const nsph_solver* S, int* n_fluid, int* n_boundary) {
     NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_num_fluid: NULL solver");
     if (n_fluid) *n_fluid = S->Nf;
-    if (n_boundary) *n_boundary = S->Nb;
+    if (n_boundary) *n_boundary = S->Nb + S->Nk;
     return NSPH_OK;
 }
 

@@ -162,9 +162,12 @@ class SPHParams:
 class SolverSPH(SolverBase):
     """Incompressible SPH (IISPH) particle step on MI355X.
 
-    Fluid particles are ACTIVE with mass > 0; ACTIVE zero-mass particles are static
-    boundary samples (walls, obstacles) with pseudo-mass rest_density * boundary_volume;
-    inactive particles are carried unchanged (solver.py:27-29).  Worlds never interact.
+    Fluid particles are ACTIVE with mass > 0; ACTIVE zero-mass particles are boundary
+    samples (walls, obstacles) with pseudo-mass rest_density * boundary_volume: static when
+    the model gives them no velocity (hashed once from the model), moving otherwise (pistons,
+    paddles: the reference integrator moves them with their velocity, and each step hashes
+    them from state_in and counts their velocity in the density change); inactive particles
+    are carried unchanged (solver.py:27-29).  Worlds never interact.
 
     Args:
         model: the Model (on a ROCm device).

@@ -52,7 +52,7 @@ typedef struct {
 } orc_gw;
 
 typedef struct {
-    int nf, nb, cap, overflow, max_count;
+    int nf, nb, nbs, cap, overflow, max_count;  /* nb boundary entries: nbs static, then moving */
     int* fid;       /* fluid ids (ascending) */
     int* fworld;    /* world per fluid-list entry */
     int* fperm;     /* sorted fluid s -> fluid-list index */
@@ -60,7 +60,8 @@ typedef struct {
     int* is_fluid;  /* per particle */
     int* nbr;       /* s * cap + k -> sorted fluid index */
     int* ncnt;
-    float* bxs;     /* sorted boundary positions */
+    float* bxs;     /* sorted boundary positions: static (model), then moving (state_in) */
+    float* bvs;     /* their velocities (0 for static) */
     int* bstart;    /* CSR of boundary neighbours per sorted fluid */
     int* blist;
     int* w_fstart;
@@ -185,7 +186,8 @@ static int scan_cells(const orc_gw* g, const int* tab, const float* pts, int s, 
  * search positions; the candidates of a particle are the points within Rs of its search
  * position (search positions of both), and its list the candidates within H of it now (current
  * positions, FP32 d2 < H^2), in candidate order, i.e. ascending sorted index. */
-static int orc_layout_build(const orc_problem* pb, const float* q_in, const float* xb_in, orc_layout* L) {
+static int orc_layout_build(const orc_problem* pb, const float* q_in, const float* qd_in, const float* xb_in,
+                            orc_layout* L) {
     memset(L, 0, sizeof(*L));
     const int N = pb->n, W = pb->n_worlds;
     L->cap = pb->max_neighbors;
@@ -214,14 +216,44 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, const floa
             nb++;
         }
     }
+    /* zero-mass particles: static boundary when the model gives them no velocity (hashed from
+     * the model's positions), moving boundary otherwise (hashed from state_in every step);
+     * statics first, each group in ascending id */
+    int nbs = 0;
+    {
+        int* mid = (int*)malloc(sizeof(int) * ((size_t)nb + 1));
+        int* mworld = (int*)malloc(sizeof(int) * ((size_t)nb + 1));
+        int nm = 0;
+        for (int k = 0; k < nb; ++k) {
+            const int id = bid[k];
+            const int rest = pb->qd0[3 * id] == 0.f && pb->qd0[3 * id + 1] == 0.f && pb->qd0[3 * id + 2] == 0.f;
+            if (rest) {
+                bid[nbs] = id;
+                bworld[nbs] = bworld[k];
+                nbs++;
+            } else {
+                mid[nm] = id;
+                mworld[nm] = bworld[k];
+                nm++;
+            }
+        }
+        for (int k = 0; k < nm; ++k) {
+            bid[nbs + k] = mid[k];
+            bworld[nbs + k] = mworld[k];
+        }
+        free(mid);
+        free(mworld);
+    }
     L->nf = nf;
     L->nb = nb;
+    L->nbs = nbs;
     for (int w = 1; w < W; ++w) L->w_fstart[w] = L->w_fstart[w - 1] + L->w_nf[w - 1];
 
     orc_gw* gws = (orc_gw*)malloc(sizeof(orc_gw) * (size_t)W);
     const size_t cells = make_grids(pb, gws);
     int* ft = (int*)calloc(2 * (cells + 1), sizeof(int));
     int* bt = (int*)calloc(2 * (cells + 1), sizeof(int));
+    int* kt = (int*)calloc(2 * (cells + 1), sizeof(int));
 
     /* search positions: per world, state_in's when every particle lies within the rebuild
      * distance of them, else the current ones */
@@ -274,22 +306,30 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, const floa
         if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell] = s;
         if (s == nf - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) ft[2 * cell + 1] = s + 1;
     }
-    /* boundary (positions of the model at creation) */
-    for (int k = 0; k < nb; ++k) {
-        const int id = bid[k];
-        kv[k].key = key_of(&gws[bworld[k]], pb->q0[3 * id], pb->q0[3 * id + 1], pb->q0[3 * id + 2]);
-        kv[k].idx = k;
-    }
-    qsort(kv, (size_t)nb, sizeof(kv_t), kv_cmp);
+    /* boundary: static from the model's positions, moving from state_in's, each group
+     * stably sorted by key with its own cell table (entries relative to the group's start) */
     L->bxs = (float*)malloc(sizeof(float) * 3 * ((size_t)nb + 1));
-    for (int s = 0; s < nb; ++s) {
-        const int id = bid[kv[s].idx];
-        L->bxs[3 * s] = pb->q0[3 * id];
-        L->bxs[3 * s + 1] = pb->q0[3 * id + 1];
-        L->bxs[3 * s + 2] = pb->q0[3 * id + 2];
-        const uint32_t cell = kv[s].key >> ORC_XSUB_BITS;
-        if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) bt[2 * cell] = s;
-        if (s == nb - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) bt[2 * cell + 1] = s + 1;
+    L->bvs = (float*)calloc(3 * ((size_t)nb + 1), sizeof(float));
+    for (int grp = 0; grp < 2; ++grp) {
+        const int first = grp == 0 ? 0 : nbs, cnt = grp == 0 ? nbs : nb - nbs;
+        const float* src = grp == 0 ? pb->q0 : q_in;
+        int* tab = grp == 0 ? bt : kt;
+        for (int k = 0; k < cnt; ++k) {
+            const int id = bid[first + k];
+            kv[k].key = key_of(&gws[bworld[first + k]], src[3 * id], src[3 * id + 1], src[3 * id + 2]);
+            kv[k].idx = k;
+        }
+        qsort(kv, (size_t)cnt, sizeof(kv_t), kv_cmp);
+        for (int s = 0; s < cnt; ++s) {
+            const int id = bid[first + kv[s].idx];
+            for (int a = 0; a < 3; ++a) {
+                L->bxs[3 * (first + s) + a] = src[3 * id + a];
+                L->bvs[3 * (first + s) + a] = (grp == 1 && qd_in) ? qd_in[3 * id + a] : 0.f;
+            }
+            const uint32_t cell = kv[s].key >> ORC_XSUB_BITS;
+            if (s == 0 || (kv[s - 1].key >> ORC_XSUB_BITS) != cell) tab[2 * cell] = s;
+            if (s == cnt - 1 || (kv[s + 1].key >> ORC_XSUB_BITS) != cell) tab[2 * cell + 1] = s + 1;
+        }
     }
     free(kv);
 
@@ -344,29 +384,35 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, const floa
         axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
         axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
         L->bstart[s] = (int)bn;
-        for (int iz = 0; iz < nz; ++iz)
-            for (int iy = 0; iy < ny; ++iy) {
-                const int row = g->cell_off + ((z0 + iz) * g->dim[1] + (y0 + iy)) * g->dim[0];
-                for (int ix = 0; ix < nx; ++ix) {
-                    const int cell = row + x0 + ix;
-                    for (int j = bt[2 * cell]; j < bt[2 * cell + 1]; ++j) {
-                        const float dx = qx - L->bxs[3 * j], dy = qy - L->bxs[3 * j + 1], dz = qz - L->bxs[3 * j + 2];
-                        const float d2 = (dx * dx + dy * dy) + dz * dz;
-                        if (!(d2 < H2)) continue;
-                        if (bn == bcap) {
-                            bcap *= 2;
-                            L->blist = (int*)realloc(L->blist, sizeof(int) * bcap);
+        for (int grp = 0; grp < 2; ++grp) {  /* static boundary, then moving */
+            const int* tab = grp == 0 ? bt : kt;
+            const int first = grp == 0 ? 0 : nbs;
+            for (int iz = 0; iz < nz; ++iz)
+                for (int iy = 0; iy < ny; ++iy) {
+                    const int row = g->cell_off + ((z0 + iz) * g->dim[1] + (y0 + iy)) * g->dim[0];
+                    for (int ix = 0; ix < nx; ++ix) {
+                        const int cell = row + x0 + ix;
+                        for (int j = first + tab[2 * cell]; j < first + tab[2 * cell + 1]; ++j) {
+                            const float dx = qx - L->bxs[3 * j], dy = qy - L->bxs[3 * j + 1],
+                                        dz = qz - L->bxs[3 * j + 2];
+                            const float d2 = (dx * dx + dy * dy) + dz * dz;
+                            if (!(d2 < H2)) continue;
+                            if (bn == bcap) {
+                                bcap *= 2;
+                                L->blist = (int*)realloc(L->blist, sizeof(int) * bcap);
+                            }
+                            L->blist[bn++] = j;
                         }
-                        L->blist[bn++] = j;
                     }
                 }
-            }
+        }
     }
     L->bstart[nf] = (int)bn;
     free(fx);
     free(fb);
     free(ft);
     free(bt);
+    free(kt);
     free(gws);
     free(bid);
     free(bworld);
@@ -375,7 +421,7 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, const floa
 
 static void orc_layout_free(orc_layout* L) {
     free(L->fid); free(L->fworld); free(L->fperm); free(L->fworld_s); free(L->is_fluid);
-    free(L->nbr); free(L->ncnt); free(L->bxs); free(L->bstart); free(L->blist);
+    free(L->nbr); free(L->ncnt); free(L->bxs); free(L->bvs); free(L->bstart); free(L->blist);
     free(L->w_fstart); free(L->w_nf); free(L->xbk);
     memset(L, 0, sizeof(*L));
 }
@@ -515,7 +561,7 @@ int orc_neighbors_brute(const float* x, const int* flags, int n, float H2, int c
 int orc_sorted_order(const orc_problem* pb, const float* q_in, const float* xb_in, int* ids_out, int* nf_out,
                      int* nb_out) {
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, xb_in, &L);
+    int rc = orc_layout_build(pb, q_in, NULL, xb_in, &L);
     if (rc) return rc;
     for (int s = 0; s < L.nf; ++s) ids_out[s] = L.fid[L.fperm[s]];
     *nf_out = L.nf;
@@ -528,7 +574,7 @@ int orc_sorted_order(const orc_problem* pb, const float* q_in, const float* xb_i
  * idx[k * n + id] like nsph_solver_neighbors; xb_in as in orc_step_f32). */
 int orc_solver_neighbors(const orc_problem* pb, const float* q_in, const float* xb_in, int* counts, int* idx) {
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, xb_in, &L);
+    int rc = orc_layout_build(pb, q_in, NULL, xb_in, &L);
     if (rc) return rc;
     for (int i = 0; i < pb->n; ++i) counts[i] = 0;
     for (int s = 0; s < L.nf; ++s) {

@@ -87,7 +87,7 @@ static REAL FN(dot3)(REAL ax, REAL ay, REAL az, REAL bx, REAL by, REAL bz) {
 /* Density (Neumaier-compensated, list then boundary order), a_ii and the boundary gradient
  * sum of every sorted fluid particle (the density half of k_sph_nbr_density). */
 static void FN(pass_density)(const FN(phys_t) * P, const orc_layout* Lp, const REAL* xs, REAL* rho, REAL* aii,
-                             REAL* gb, REAL dt2, int nthreads) {
+                             REAL* gb, REAL* kvb, REAL dt2, int nthreads) {
     const orc_layout L = *Lp;
     const int Nf = L.nf;
     const int* nbr = L.nbr;
@@ -112,7 +112,8 @@ static void FN(pass_density)(const FN(phys_t) * P, const orc_layout* Lp, const R
             gfz += mj * gz;
             S += mj * FN(dot3)(gx, gy, gz, gx, gy, gz);
         }
-        REAL gbx = 0, gby = 0, gbz = 0;
+        /* boundary: static entries, then moving ones, which also add v_b . gradW_ib to kvb */
+        REAL gbx = 0, gby = 0, gbz = 0, kv = 0;
         for (int k = L.bstart[s]; k < L.bstart[s + 1]; ++k) {
             const int b = L.blist[k];
             const REAL dx = xi - (REAL)L.bxs[3 * b], dy = yi - (REAL)L.bxs[3 * b + 1],
@@ -121,10 +122,14 @@ static void FN(pass_density)(const FN(phys_t) * P, const orc_layout* Lp, const R
             REAL Wv, gf;
             FN(kern_wg)(Pw, r2, &Wv, &gf);
             FN(neumaier)(&sum, &cmp, Pw->psi * Wv);
-            gbx += gf * dx;
-            gby += gf * dy;
-            gbz += gf * dz;
+            const REAL gx = gf * dx, gy = gf * dy, gz = gf * dz;
+            gbx += gx;
+            gby += gy;
+            gbz += gz;
+            if (b >= L.nbs)
+                kv += FN(dot3)((REAL)L.bvs[3 * b], (REAL)L.bvs[3 * b + 1], (REAL)L.bvs[3 * b + 2], gx, gy, gz);
         }
+        kvb[s] = kv;
         const REAL r = sum + cmp;
         const REAL Gx = gfx + Pw->psi * gbx, Gy = gfy + Pw->psi * gby, Gz = gfz + Pw->psi * gbz;
         const REAL num = FN(dot3)(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
@@ -180,7 +185,7 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
                              int* iters_out, double* err_out, int nthreads) {
     const int N = pb->n, W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, xb_in, &L);
+    int rc = orc_layout_build(pb, q_in, qd_in, xb_in, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
@@ -199,6 +204,7 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
     REAL* rho = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* aii = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* gb = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* kvb = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* av = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
     REAL* va = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
     REAL* src = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
@@ -224,7 +230,7 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
     const int* ncnt = L.ncnt;
     const int cap = L.cap;
 
-    FN(pass_density)(P, &L, xs, rho, aii, gb, dt2, nthreads);
+    FN(pass_density)(P, &L, xs, rho, aii, gb, kvb, dt2, nthreads);
     FN(pass_viscosity)(P, &L, xs, vs, rho, av, va, rdt, nthreads);
     /* source term, warm start */
 #pragma omp parallel for num_threads(nthreads) schedule(static)
@@ -241,7 +247,8 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
             sum += xs[4 * j + 3] * FN(dot3)(vxi - va[3 * j], vyi - va[3 * j + 1], vzi - va[3 * j + 2],
                                             gf * dx, gf * dy, gf * dz);
         }
-        sum += Pw->psi * FN(dot3)(vxi, vyi, vzi, gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]);
+        /* boundary: psi sum_b (v_i - v_b) . gradW_ib */
+        sum += Pw->psi * (FN(dot3)(vxi, vyi, vzi, gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]) - kvb[s]);
         src[s] = (Pw->rho0 - rho[s]) - rdt * sum;
         REAL p0 = 0;
         const REAL pp = (REAL)pprev[L.fperm[s]];
@@ -430,7 +437,7 @@ static int FN(orc_step_impl)(const orc_problem* pb, const float* q_in, const flo
     for (int w = 0; w < W; ++w)
         if (L.w_nf[w] > 0 && conv[w] != 1) status = 6;
     if (L.overflow) status = 3;
-    free(P); free(xs); free(vs); free(rho); free(aii); free(gb); free(av); free(va); free(src);
+    free(P); free(xs); free(vs); free(rho); free(aii); free(gb); free(kvb); free(av); free(va); free(src);
     free(p); free(pr); free(ap); free(e); free(conv); free(iters); free(errw);
     orc_layout_free(&L);
     return status;
@@ -445,7 +452,7 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
                                  double* s_o, double* Ap_o, double* ap_o) {
     const int W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, NULL, &L);
+    int rc = orc_layout_build(pb, q_in, qd_in, NULL, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
@@ -457,6 +464,7 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
     REAL* xs = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)(Nf + 1));
     REAL* rho = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* gb = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
+    REAL* kvb = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* pr = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
     REAL* ap = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
     REAL* va = (REAL*)malloc(sizeof(REAL) * 3 * (size_t)(Nf + 1));
@@ -480,7 +488,7 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
             gfz += xs[4 * j + 3] * gf * dz;
             S += xs[4 * j + 3] * (gf * gf) * ((dx * dx + dy * dy) + dz * dz);
         }
-        REAL gbx = 0, gby = 0, gbz = 0;
+        REAL gbx = 0, gby = 0, gbz = 0, kv = 0;
         for (int k = L.bstart[s]; k < L.bstart[s + 1]; ++k) {
             const int b = L.blist[k];
             const REAL dx = xs[4 * s] - (REAL)L.bxs[3 * b], dy = xs[4 * s + 1] - (REAL)L.bxs[3 * b + 1],
@@ -491,7 +499,11 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
             gbx += gf * dx;
             gby += gf * dy;
             gbz += gf * dz;
+            if (b >= L.nbs)
+                kv += FN(dot3)((REAL)L.bvs[3 * b], (REAL)L.bvs[3 * b + 1], (REAL)L.bvs[3 * b + 2], gf * dx, gf * dy,
+                               gf * dz);
         }
+        kvb[s] = kv;
         rho[s] = sum + cmp;
         gb[3 * s] = gbx;
         gb[3 * s + 1] = gby;
@@ -519,7 +531,8 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
             ay += c * (gf * dy);
             az += c * (gf * dz);
         }
-        sum += Pw->psi * FN(dot3)(va[3 * s], va[3 * s + 1], va[3 * s + 2], gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]);
+        sum += Pw->psi * (FN(dot3)(va[3 * s], va[3 * s + 1], va[3 * s + 2], gb[3 * s], gb[3 * s + 1], gb[3 * s + 2]) -
+                          kvb[s]);
         const REAL cb = Pw->psi * pr[s];
         ap[3 * s] = -(ax + cb * gb[3 * s]);
         ap[3 * s + 1] = -(ay + cb * gb[3 * s + 1]);
@@ -541,7 +554,7 @@ static int FN(orc_operator_impl)(const orc_problem* pb, const float* q_in, const
         Ap_o[k] = (double)(dt2 * sum);
         for (int a = 0; a < 3; ++a) ap_o[3 * k + a] = (double)ap[3 * s + a];
     }
-    free(P); free(xs); free(rho); free(gb); free(pr); free(ap); free(va);
+    free(P); free(xs); free(rho); free(gb); free(kvb); free(pr); free(ap); free(va);
     orc_layout_free(&L);
     return 0;
 }
@@ -552,7 +565,7 @@ static int FN(orc_viscosity_impl)(const orc_problem* pb, const float* q_in, cons
                                   double* rho_o) {
     const int W = pb->n_worlds;
     orc_layout L;
-    int rc = orc_layout_build(pb, q_in, NULL, &L);
+    int rc = orc_layout_build(pb, q_in, qd_in, NULL, &L);
     if (rc) return rc;
     const int Nf = L.nf;
     FN(phys_t)* P = (FN(phys_t)*)calloc((size_t)W, sizeof(FN(phys_t)));
@@ -574,7 +587,8 @@ static int FN(orc_viscosity_impl)(const orc_problem* pb, const float* q_in, cons
         }
         xs[4 * s + 3] = (REAL)pb->mass[id];
     }
-    FN(pass_density)(P, &L, xs, rho, aii, gb, (REAL)0, 1);
+    REAL* kvb = (REAL*)malloc(sizeof(REAL) * (size_t)(Nf + 1));
+    FN(pass_density)(P, &L, xs, rho, aii, gb, kvb, (REAL)0, 1);
     FN(pass_viscosity)(P, &L, xs, vs, rho, av, va, (REAL)0, 1);
     for (int i = 0; i < 3 * pb->n; ++i) av_o[i] = 0.0;
     for (int i = 0; i < pb->n; ++i) rho_o[i] = 0.0;
@@ -583,7 +597,7 @@ static int FN(orc_viscosity_impl)(const orc_problem* pb, const float* q_in, cons
         for (int a = 0; a < 3; ++a) av_o[3 * id + a] = (double)av[3 * s + a];
         rho_o[id] = (double)rho[s];
     }
-    free(P); free(xs); free(vs); free(rho); free(aii); free(gb); free(av); free(va);
+    free(P); free(xs); free(vs); free(rho); free(aii); free(gb); free(kvb); free(av); free(va);
     orc_layout_free(&L);
     return 0;
 }
