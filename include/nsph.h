@@ -86,6 +86,22 @@ int nsph_grid_neighbors(const nsph_grid* grid, const int* d_flags, float radius,
 int nsph_grid_query(const nsph_grid* grid, const float* d_q, int nq, float radius,
                     int max_neighbors, int* d_counts, int* d_idx);
 int nsph_grid_destroy(nsph_grid* grid);
+/* Device view of the last build, for in-kernel queries (include/nsph_device.h: point_id,
+ * query, query_next, replacing hash_grid_point_id / hash_grid_query / hash_grid_query_next
+ * of kernels_contact.py:54-73 and xpbd/kernels.py:244-264).  Valid until the next build. */
+typedef struct nsph_grid_device {
+    const int* cells;          /* 2 per cell: first and end sorted position; cell index
+                                  (cz * dim_y + cy) * dim_x + cx of the periodic hash */
+    const unsigned int* ids;   /* point id at each sorted position                        */
+    const float* points;       /* 4 per sorted position: x, y, z of the build, 0          */
+    int dim[3];
+    float inv_cell;            /* 1 / cell width (the build radius)                       */
+    int n;                     /* points of the build                                     */
+} nsph_grid_device;
+int nsph_grid_device_view(const nsph_grid* grid, nsph_grid_device* out);
+/* A consumer written on nsph_grid_device.h: d_counts[k] = number of grid points within
+ * `radius` of query point k (FP32 (dx*dx + dy*dy) + dz*dz < radius^2). */
+int nsph_grid_count(const nsph_grid* grid, const float* d_q, int nq, float radius, int* d_counts);
 
 /* ---- the reference's particle kernels on the HIP grid ------------------------------ */
 

@@ -128,6 +128,8 @@ SIGNATURES: dict[str, list] = {
     "nsph_grid_neighbors": [_vp, _vp, _f, _i, _vp, _vp],
     "nsph_grid_query": [_vp, _vp, _i, _f, _i, _vp, _vp],
     "nsph_grid_destroy": [_vp],
+    "nsph_grid_device_view": [_vp, _vp],
+    "nsph_grid_count": [_vp, _vp, _i, _f, _vp],
     "nsph_solver_create": [_vp, C.POINTER(ModelDesc), C.POINTER(SphParams), C.POINTER(_vp)],
     "nsph_solver_step": [_vp, C.POINTER(StateIn), C.POINTER(StateOut), _f, _i, C.POINTER(StepReport)],
     "nsph_solver_report": [_vp, C.POINTER(StepReport)],

@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "../../include/nsph_device.h"
 
 namespace nsph {
 
@@ -131,6 +132,21 @@ __global__ void k_grid_query(GridWorld g, const int2* __restrict__ table,
                });
     counts[k] = c;
     if (c > max_nbrs) atomicOr(overflow, 1);
+}
+
+// Points within r of each query point, through the public in-kernel query (nsph_device.h).
+__global__ void k_grid_count(nsph_grid_device g, const float* __restrict__ q, int nq, float r, float r2max,
+                             int* __restrict__ counts) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nq) return;
+    const float x = q[3 * k], y = q[3 * k + 1], z = q[3 * k + 2];
+    nsph_grid_query_t it = nsph_grid_query_begin(&g, x, y, z, r);
+    int id, pos, c = 0;
+    while (nsph_grid_query_next(&g, &it, &id, &pos)) {
+        const float dx = x - g.points[4 * pos], dy = y - g.points[4 * pos + 1], dz = z - g.points[4 * pos + 2];
+        if ((dx * dx + dy * dy) + dz * dz < r2max) ++c;
+    }
+    counts[k] = c;
 }
 
 }  // namespace nsph
@@ -359,6 +375,31 @@ int nsph_grid_query(const nsph_grid* g, const float* d_q, int nq, float radius, 
     int ov = 0;
     NSPH_TRY(read_overflow(g, &ov));
     NSPH_CHECK(!ov, NSPH_ERR_CAPACITY, "nsph_grid_query: a list exceeded max_neighbors=%d", max_neighbors);
+    return NSPH_OK;
+}
+
+int nsph_grid_device_view(const nsph_grid* g, nsph_grid_device* out) {
+    NSPH_CHECK(g != nullptr && out != nullptr, NSPH_ERR_INVALID, "nsph_grid_device_view: NULL argument");
+    NSPH_CHECK(g->built, NSPH_ERR_NOT_BUILT, "nsph_grid_device_view: grid not built");
+    out->cells = bp<int>(g->table);
+    out->ids = g->perm;
+    out->points = bp<float>(g->xs);
+    for (int a = 0; a < 3; ++a) out->dim[a] = g->dim[a];
+    out->inv_cell = g->g.inv_w;
+    out->n = g->n;
+    return NSPH_OK;
+}
+
+int nsph_grid_count(const nsph_grid* g, const float* d_q, int nq, float radius, int* d_counts) {
+    NSPH_CHECK(g != nullptr && d_counts != nullptr, NSPH_ERR_INVALID, "nsph_grid_count: NULL argument");
+    NSPH_CHECK(radius > 0.f && nq >= 0, NSPH_ERR_INVALID, "nsph_grid_count: bad argument");
+    nsph_grid_device view;
+    NSPH_TRY(nsph_grid_device_view(g, &view));
+    if (nq == 0) return NSPH_OK;
+    NSPH_CHECK(d_q != nullptr, NSPH_ERR_INVALID, "nsph_grid_count: NULL query points");
+    hipLaunchKernelGGL(k_grid_count, dim3((nq + 255) / 256), dim3(256), 0, g->ctx->stream, view, d_q, nq, radius,
+                       radius * radius, d_counts);
+    NSPH_HIP(hipGetLastError());
     return NSPH_OK;
 }
 

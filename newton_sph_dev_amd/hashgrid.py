@@ -100,6 +100,16 @@ class HashGrid:
                N.ptr(counts), N.ptr(idx))
         return counts, idx
 
+    def count(self, points: torch.Tensor, radius: float) -> torch.Tensor:
+        """Number of grid points within radius of each query point (the library's consumer
+        of the in-kernel query of include/nsph_device.h)."""
+        self._require()
+        q = points.reshape(-1, 3).contiguous()
+        nq = int(q.shape[0])
+        counts = torch.empty(nq, dtype=torch.int32, device=self.device)
+        N.call("nsph_grid_count", self._handle, N.ptr(q), nq, float(radius), N.ptr(counts))
+        return counts
+
     def __del__(self):
         h = getattr(self, "_handle", None)
         if h is not None and N._lib is not None:

@@ -86,6 +86,28 @@ def test_hashgrid_rebuild_and_aliasing():
         assert _lists_as_sets(counts.cpu().numpy(), idx.cpu().numpy(), True) == _lists_as_sets(bc, bi, False)
 
 
+def test_hashgrid_device_query_counts():
+    """The in-kernel query of include/nsph_device.h (through its consumer nsph_grid_count) finds
+    what the whole-array query finds: counts equal nsph_grid_query's and brute force, also on a
+    3^3 periodic grid that wraps the cloud several times and for query points outside it."""
+    rng = np.random.default_rng(9)
+    pts = (rng.random((5000, 3)) * 0.9).astype(np.float32)
+    qs = (rng.random((700, 3)) * 1.0 - 0.05).astype(np.float32)
+    for dims, r in (((16, 16, 16), 0.05), ((3, 3, 3), 0.07)):
+        _print(f"hashgrid device query: 5000 points, 700 queries, dims {dims}, radius {r}")
+        g = HashGrid(*dims, device="cuda:0")
+        g.reserve(5000)
+        g.build(torch.from_numpy(pts).cuda(), r)
+        got = g.count(torch.from_numpy(qs).cuda(), r).cpu().numpy()
+        cq, _ = g.query(torch.from_numpy(qs).cuda(), r, max_neighbors=256)
+        d = qs[:, None, :] - pts[None, :, :]
+        d2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        brute = np.count_nonzero(d2 < np.float32(r) * np.float32(r), axis=1)
+        assert np.array_equal(got, cq.cpu().numpy())
+        assert np.array_equal(got, brute)
+        assert got.max() > 3
+
+
 @pytest.mark.parametrize("dims", [(12, 10, 14), (20, 16, 24)])
 def test_step_small_dam_break_parity(dims):
     cfg = scenes.small_dam_break(*dims)

@@ -134,6 +134,22 @@ def test_abi_exports_every_declared_symbol():
         assert name in N.SIGNATURES or name in ("nsph_last_error", "nsph_kernel_name"), name
 
 
+def test_device_header_compiles_as_c_and_hip():
+    """include/nsph_device.h: its C part (the view struct) for a host C compiler, its device
+    query functions for hipcc."""
+    import shutil
+    import subprocess
+
+    hdr = ROOT / "include" / "nsph_device.h"
+    subprocess.run(["gcc", "-fsyntax-only", "-Wall", "-Werror", "-x", "c", str(hdr)], check=True)
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not Path(hipcc).exists():
+        pytest.skip("hipcc not available")
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-fsyntax-only", "-Wall", "-Werror",
+                    "-Wno-unused-command-line-argument", "-x", "hip",
+                    str(hdr)], check=True)
+
+
 def test_abi_errors_without_device():
     """Without a visible GPU the context cannot be created; the error is a status code plus
     a message, never a crash."""
