@@ -35,12 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # What limits each kernel class, from rocprofv3 counters on MI355X (profiles/pmc_dam_break_1m.md):
 # VALU busy (SQ_ACTIVE_INST_VALU) and L1 line accesses per cycle (TCP_TOTAL_CACHE_ACCESSES).
 LIMITERS = {
-    "neighbors_density": "valu (VALUBusy ~100 %; HBM traffic ~1 TB/s)",
-    "pressure_update": "l1 line accesses (~1 per CU cycle; TA 78 % busy, VALU 58 %)",
-    "advect": "l1 line accesses and valu (TA 76 %, VALU 77 %)",
-    "source": "l1 line accesses (TA ~76 %, VALU 64 %)",
-    "pressure_acc": "valu (90 %)",
-    "final": "l1 line accesses (VALU 56 %)",
+    "neighbors_density": "valu (~100 % busy at 4 cycles per wave64 instruction; TA 60 %, L1 0.51 lines/cycle)",
+    "pressure_update": "l1 line accesses (0.99 per CU cycle; TA 77 %, VALU 57 %)",
+    "advect": "l1 line accesses and valu (0.92 per CU cycle; TA 76 %, VALU 74 %)",
+    "source": "l1 line accesses (0.95 per CU cycle; TA 75 %, VALU 60 %)",
+    "pressure_acc": "l1 line accesses and valu (0.92 per CU cycle; VALU 88 %)",
+    "final": "l1 line accesses and valu (0.83 per CU cycle; VALU 80 %)",
 }
 PROBE_STEPS = 10  # eager steps with every kernel class timed, before the timed steps
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
@@ -59,11 +59,11 @@ def measured_traffic(workload: str, kernel: str):
 
 def sort_passes(params) -> int:
     """Radix passes of the cell sort (8-bit digits, or 9-bit where that saves a pass): keys
-    count the cells of every world's box (cell width H (1 + 1/128), at least 3 per axis)
-    times 4 x-slabs per cell."""
+    count the cells of every world's box (cell width H (1 + skin) (1 + 1/128), at least 3 per
+    axis) times 4 x-slabs per cell."""
     total = 0
     for w in range(params.n_worlds):
-        cw = params.smoothing_length[w] * (1.0 + 1.0 / 128.0)
+        cw = params.smoothing_length[w] * (1.0 + params.skin) * (1.0 + 1.0 / 128.0)
         d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / cw) + 1.0, 3.0)
         total += int(np.prod(d))
     bits = max(int(np.ceil(np.log2(max(total * 4, 2)))), 1)
