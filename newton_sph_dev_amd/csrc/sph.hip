@@ -588,21 +588,21 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
 }
 
 // Candidates j within r of (qx,qy,qz) in the order z, y, then ascending sorted index;
-// calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).  Positions are fetched RV_BATCH
-// at a time.
+// calls f(j, dx, dy, dz, d2) for d2 < r2max (d = q - x_j).  Positions are fetched B at a time
+// (RV_BATCH by default).
 #define RV_BATCH 4
 #define XS_PAD 8  // positions readable past the last fluid particle
-template <class F>
+template <int B = RV_BATCH, class F>
 __device__ inline void rows_visit(const GridWorld& g, const int2* __restrict__ table, const int* __restrict__ cend,
                                   const float4* __restrict__ xs, float qx, float qy, float qz,
                                   float r, float r2max, F f) {
     rows_for_each(g, table, cend, qx, qy, qz, r, [&](int lo, int hi) {
-        for (int j = lo; j < hi; j += RV_BATCH) {
-            float4 p[RV_BATCH];
+        for (int j = lo; j < hi; j += B) {
+            float4 p[B];
 #pragma unroll
-            for (int u = 0; u < RV_BATCH; ++u) p[u] = xs[min(j + u, hi - 1)];
+            for (int u = 0; u < B; ++u) p[u] = xs[min(j + u, hi - 1)];
 #pragma unroll
-            for (int u = 0; u < RV_BATCH; ++u) {
+            for (int u = 0; u < B; ++u) {
                 if (j + u < hi) {
                     const float dx = qx - p[u].x, dy = qy - p[u].y, dz = qz - p[u].z;
                     const float d2 = (dx * dx + dy * dy) + dz * dz;
@@ -966,6 +966,7 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     density_finish(D, P, s, xi.w, sum, cmp, gfx, gfy, gfz, S, 0.f, 0.f, 0.f, P.H2, 0, dt2);
 }
 
+#define BND_BATCH 8
 // Boundary terms of the queued particles, continuing their compensated density sum in the
 // boundary candidate order, then the same finish as the neighbour kernel.  Grid-stride over
 // the queue (its length is known on the device only).
@@ -982,7 +983,7 @@ __global__ __launch_bounds__(256) void k_sph_bnd_density(SphDev D, float dt2) {
         float sum = a.x, cmp = a.y;
         float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
         int nb = 0;
-        rows_visit(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+        rows_visit<BND_BATCH>(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
                    [&](int, float dx, float dy, float dz, float r2) {
                        float W, gf;
                        kern_wg(P, r2, &W, &gf);
@@ -996,7 +997,7 @@ __global__ __launch_bounds__(256) void k_sph_bnd_density(SphDev D, float dt2) {
         // moving boundary particles after the static ones, same terms, plus v_b . gradW_ib
         float kvb = 0.f;
         if (D.kxs != nullptr)
-            rows_visit(g, D.ktable, D.kcend, D.kxs, xi.x, xi.y, xi.z, P.H, P.H2,
+            rows_visit<BND_BATCH>(g, D.ktable, D.kcend, D.kxs, xi.x, xi.y, xi.z, P.H, P.H2,
                        [&](int j, float dx, float dy, float dz, float r2) {
                            float W, gf;
                            kern_wg(P, r2, &W, &gf);
