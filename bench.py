@@ -59,12 +59,13 @@ def measured_traffic(workload: str, kernel: str):
 
 def sort_passes(params) -> int:
     """Radix passes of the cell sort (8-bit digits, or 9-bit where that saves a pass): keys
-    count the cells of every world's box (cell width H (1 + skin) (1 + 1/128), at least 3 per
-    axis) times 4 x-slabs per cell."""
+    count the cells of every world's box (cell width w = H (1 + skin) (1 + 1/128) along x and
+    y, w / 2 along z, at least 3 per axis) times 4 x-slabs per cell."""
     total = 0
     for w in range(params.n_worlds):
         cw = params.smoothing_length[w] * (1.0 + params.skin) * (1.0 + 1.0 / 128.0)
-        d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / cw) + 1.0, 3.0)
+        widths = np.array([cw, cw, 0.5 * cw])
+        d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / widths) + 1.0, 3.0)
         total += int(np.prod(d))
     bits = max(int(np.ceil(np.log2(max(total * 4, 2)))), 1)
     return min((bits + 7) // 8, (bits + 8) // 9)

@@ -66,7 +66,8 @@ struct GridWorld {
     int dim[3];
     int wrap;
     int cell_off;
-    int pad[3];
+    float inv_t;  // SPH grids: inverse height of the z cells, 2 inv_w (half-height cells)
+    int pad[2];
 };
 
 __host__ __device__ inline int cell_coord(float x, float lo, float inv_w, int dim, int wrap) {

@@ -116,14 +116,14 @@ struct SphDev {
     const int2* btable;
     const int* bcend;
     const float4* bxs;
-    const unsigned char* breach;  // per cell: 1 if a boundary particle lies in its 3x3x3 block
+    const unsigned char* breach;  // per cell: 1 if a boundary particle lies in its 3x3x5 block
     // moving boundary (zero-mass particles with a velocity), hashed every step from state_in;
     // kxs == NULL when there is none
     const int2* ktable;
     const int* kcend;
     const float4* kxs;
     const float4* kvs;
-    const unsigned char* kbreach;  // per cell: 1 if a moving boundary particle lies in its 3x3x3 block
+    const unsigned char* kbreach;  // per cell: 1 if a moving boundary particle lies in its 3x3x5 block
     float* kvb;  // per sorted fluid particle: sum_b v_b . gradW_ib over moving boundary particles
     // fluid grid
     int2* ftable;
@@ -209,10 +209,19 @@ __device__ inline bool chunk_slot_sub(const SphDev& D, int nsub, int* w, int* s)
 
 // ---- build ------------------------------------------------------------------------------
 //
+// Cells are w wide along x and y and w / 2 along z (inverse width inv_t = 2 inv_w, so the z
+// coordinate in half cells is exactly twice the full-cell one): a row of cells (cy, cz) is a
+// slab along x one particle spacing high at the support H = 2 spacings, so the 64
+// consecutive sorted particles of a wave lie in one or two such rows and their neighbour
+// lists run through the same rows in step, which keeps the number of distinct cache lines of
+// a wave-wide neighbour load low (about a third fewer than with cubic cells on a lattice).
+// A query of radius r <= w spans at most 3 cells along x and y and 5 along z.
 // Sort keys split each cell into NSPH_XSUB slabs along x:
 //   key = (cell_off + (cz*dim_y + cy)*dim_x) * XSUB + gx,   gx = clamp(floor(XSUB f_x), 0, XSUB dim_x - 1)
 // with f_x = (x - lo_x) inv_w, the same FP32 value the cell coordinate is taken from, so
-// gx >> XSUB_BITS is the cell's cx.  Sorted order is therefore (world, cz, cy, cx, slab, id).
+// gx >> XSUB_BITS is the cell's cx, cy = clamp(floor((y - lo_y) inv_w)) and
+// cz = clamp(floor((z - lo_z) inv_t)).
+// Sorted order is therefore (world, cz, cy, cx, slab, id).
 // The cell table keeps one int2 per cell: x = first sorted index of the cell, y = the counts
 // of its XSUB slabs packed as bytes (slab k in bits 8k..8k+7), 0 for an empty cell.  A query
 // can then clip each row of cells to the x-extent of its sphere at slab resolution.  A cell
@@ -235,7 +244,7 @@ __host__ __device__ inline int slab_coord(float x, float lo, float inv_w, int di
 __host__ __device__ inline uint32_t slab_key(const GridWorld& g, float x, float y, float z) {
     const int gx = slab_coord(x, g.lo[0], g.inv_w, g.dim[0]);
     const int cy = cell_coord(y, g.lo[1], g.inv_w, g.dim[1], 0);
-    const int cz = cell_coord(z, g.lo[2], g.inv_w, g.dim[2], 0);
+    const int cz = cell_coord(z, g.lo[2], g.inv_t, g.dim[2], 0);
     return (uint32_t)(g.cell_off + (cz * g.dim[1] + cy) * g.dim[0]) * NSPH_XSUB + (uint32_t)gx;
 }
 
@@ -515,17 +524,20 @@ __device__ inline int2 row_slab_range(const int2* __restrict__ c, int nx, int x0
 
 // Rows of the query sphere, in the order z, y: scan(lo, hi) is called with the sorted range
 // [lo, hi) of each row (y0 + iy, z0 + iz) of the (clamped) query box that the sphere meets,
-// k = 3 iz + iy ascending.  r <= the cell width, so the box has at most 3 x 3 rows of at
-// most 3 cells.  Each row is clipped to the sphere: with gy, gz the distances (in cells)
-// from the query to the row's y and z extent (border rows extend to infinity, as clamping
-// puts outside points there), the row is skipped when gy^2 + gz^2 >= rc^2 and otherwise only
-// the slabs meeting [f_x - hx, f_x + hx], hx = sqrt(rc^2 - gy^2 - gz^2), are kept.  Clipping
-// drops only points farther than r, so the accepted points and their order are those of the
-// unclipped rows.  rc = r/w (1 + 1/64) + 1/64 cells covers the FP32 rounding of the cell
-// coordinates (at most 2^-8 cells per axis while |x - lo| < 2^14 cell widths).
+// k = 3 iz + iy ascending.  r <= the cell width w, so the box has at most 3 x 5 rows (z cells
+// are w / 2 high) of at most 3 cells.  Each row is clipped to the sphere: with gy, gz the
+// distances (in cell widths) from the query to the row's y and z extent (border rows extend
+// to infinity, as clamping puts outside points there), the row is skipped when
+// gy^2 + gz^2 >= rc^2 and otherwise only the slabs meeting [f_x - hx, f_x + hx],
+// hx = sqrt(rc^2 - gy^2 - gz^2), are kept.  Clipping drops only points farther than r, so the
+// accepted points and their order are those of the unclipped rows.  rc = r/w (1 + 1/64) + 1/64
+// cells covers the FP32 rounding of the cell coordinates (at most 2^-8 cells per axis while
+// |x - lo| < 2^14 cell widths; the z coordinate in half cells is exactly twice the full-cell
+// one, so its distances halve exactly).
 // The row loop stays rolled (one copy of the set-up and of the caller's scan keeps the
 // kernels small enough for the instruction cache); the table entries of row k + 1 are
 // loaded before row k is scanned, so their latency hides behind the scan.
+#define NSPH_ZROWS 5  // z cells a query spans at most (3 along x and y)
 template <class Scan>
 __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict__ table,
                                      const int* __restrict__ cend, float qx, float qy, float qz,
@@ -534,12 +546,13 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
     int x0, nx, y0, ny, z0, nz;
     axis_range(qx, rp, g.lo[0], g.inv_w, g.dim[0], 0, &x0, &nx);
     axis_range(qy, rp, g.lo[1], g.inv_w, g.dim[1], 0, &y0, &ny);
-    axis_range(qz, rp, g.lo[2], g.inv_w, g.dim[2], 0, &z0, &nz);
-    const float fx = (qx - g.lo[0]) * g.inv_w, fy = (qy - g.lo[1]) * g.inv_w, fz = (qz - g.lo[2]) * g.inv_w;
+    axis_range(qz, rp, g.lo[2], g.inv_t, g.dim[2], 0, &z0, &nz);
+    const float fx = (qx - g.lo[0]) * g.inv_w, fy = (qy - g.lo[1]) * g.inv_w, fz = (qz - g.lo[2]) * g.inv_t;
     const float rcell = r * g.inv_w;
     const float rc = rcell + rcell * (1.0f / 64.0f) + (1.0f / 64.0f);
     const float rc2 = rc * rc;
     const int xa = x0 * NSPH_XSUB, xb = (x0 + nx) * NSPH_XSUB - 1;
+    // distance from v to cell c's extent, in cells of that axis
     auto gap = [](float v, int c, int dim) {
         const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
         return fmaxf(fmaxf(e0 - v, v - e1), 0.0f);
@@ -548,7 +561,7 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
     auto load_row = [&](int k, int2 c[3], int* fa, int* fb, int* row) {
         const int iz = k / 3, iy = k - 3 * (k / 3);
         if (!(iz < nz && iy < ny)) return false;
-        const float gy = gap(fy, y0 + iy, g.dim[1]), gz = gap(fz, z0 + iz, g.dim[2]);
+        const float gy = gap(fy, y0 + iy, g.dim[1]), gz = 0.5f * gap(fz, z0 + iz, g.dim[2]);
         const float h2 = rc2 - gy * gy - gz * gz;
         if (!(h2 > 0.0f)) return false;
         // the 1-ulp hardware root: the 1/64-cell pad of rc covers its error
@@ -565,14 +578,15 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
         for (int ix = 0; ix < 3; ++ix) c[ix] = table[*row + min(ix, nx - 1)];
         return true;
     };
+    constexpr int K = 3 * NSPH_ZROWS;
     int2 c[3];
     int fa = 0, fb = 0, row = 0;
     bool ok = load_row(0, c, &fa, &fb, &row);
 #pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < K; ++k) {
         int2 cn[3];
         int fan = 0, fbn = 0, rown = 0;
-        const bool okn = k < 8 && load_row(k + 1, cn, &fan, &fbn, &rown);
+        const bool okn = k < K - 1 && load_row(k + 1, cn, &fan, &fbn, &rown);
         // every lane calls scan (an empty range for a skipped row), so a scan may exchange
         // data between lanes
         int2 rr = make_int2(0, 0);
@@ -635,106 +649,42 @@ __device__ inline int row16_max(int v) {
     return v;
 }
 
-// Accept bit of candidate (px, py, pz) held by lane U of the quad, shifted into m.
-template <int U>
-__device__ inline void quad_test(uint32_t& m, float px, float py, float pz, float qx, float qy, float qz,
-                                 float r2max) {
-    constexpr int C = QPERM(U, U, U, U);
-    const float dx = qx - quad_perm<C>(px), dy = qy - quad_perm<C>(py), dz = qz - quad_perm<C>(pz);
-    const float d2 = (dx * dx + dy * dy) + dz * dz;
-    m = m + m + (d2 < r2max ? 1u : 0u);
-}
-
 // rows_visit for callers that need only the accepted indices, in the same order: each row
 // range is tested in windows of 32 candidates into an accept bitmask, and f(j) is then
 // called for the accepted candidates in ascending order.  The per-candidate work of a wave
 // no longer includes the caller's (divergent) accept code.  Candidate t of a window of n
-// enters the mask at bit n4 - 1 - t (m = 2m + accept; n4 = n rounded up to the batch: 4 per
-// lane, 8 per quad), so each test costs a compare and an add-with-carry; a window reads up to
-// XS_PAD - 1 positions past its range, which xs must hold (padding) and which are masked off.
-//
-// The four lanes of a quad are consecutive sorted particles, nearly always of one cell or
-// of neighbouring cells of one cell row, so their ranges of a row overlap.  When every
-// quad's ranges of the row overlap (their union is at most 12 points longer than the
-// longest), the quad scans the union together: lane u of the quad loads candidate 4k + u
-// and each lane reads the four candidates from the quad's registers, so one load brings
-// four candidates instead of one.  Each lane then keeps the bits of its own range, so the
-// accepted indices and their order are those of the per-lane scan.  The quad loads two
-// groups of four per step, so two loads are in flight.  A wave with inactive
-// lanes, or a row where some quad's ranges are disjoint (a quad that straddles two cell
-// rows), scans per lane.
+// enters the mask at bit n4 - 1 - t (m = 2m + accept; n4 = n rounded up to the batch), so
+// each test costs a compare and an add-with-carry; a window reads up to XS_PAD - 1 positions
+// past its range, which xs must hold (padding) and which are masked off.  The lanes of a wave
+// are consecutive particles of one or two cell rows, so the candidates of neighbouring lanes
+// are neighbouring positions and the per-lane loads share cache lines.
 template <class F>
 __device__ inline void rows_visit_ids(const GridWorld& g, const int2* __restrict__ table,
                                       const int* __restrict__ cend,
                                       const float4* __restrict__ xs, float qx, float qy, float qz,
                                       float r, float r2max, F f) {
-    const bool full = __builtin_amdgcn_read_exec() == ~0ull;
-    const int lq = (int)(__lane_id() & 3u);
-    auto extract = [&](uint32_t m, int base, int n4) {
-        while (m != 0u) {
-            const int b = 31 - __builtin_clz(m);
-            m ^= 1u << b;
-            f(base + (n4 - 1 - b));
-        }
-    };
     rows_for_each(g, table, cend, qx, qy, qz, r, [&](int lo, int hi) {
-        bool coop = false;
-        int ulo = 0, uhi = 0;
-        if (full) {
-            const bool has = lo < hi;
-            int a = has ? lo : 0x7fffffff, b = has ? hi : -1, len = has ? hi - lo : 0;
-            a = min(a, quad_perm<QPERM(1, 0, 3, 2)>(a));
-            a = min(a, quad_perm<QPERM(2, 3, 0, 1)>(a));
-            b = max(b, quad_perm<QPERM(1, 0, 3, 2)>(b));
-            b = max(b, quad_perm<QPERM(2, 3, 0, 1)>(b));
-            len = max(len, quad_perm<QPERM(1, 0, 3, 2)>(len));
-            len = max(len, quad_perm<QPERM(2, 3, 0, 1)>(len));
-            const bool tight = len == 0 || b - a <= len + 12;
-            coop = __ballot(!tight) == 0ull;
-            ulo = len == 0 ? 0 : a;
-            uhi = len == 0 ? 0 : b;
-        }
-        if (coop) {
-            // quad-uniform trip counts: the quad's lanes stay active together
-            for (int base = ulo; base < uhi; base += 32) {
-                const int n = min(uhi - base, 32);
-                uint32_t m = 0u;
-                int n4 = 0;
-                for (; n4 < n; n4 += 8) {
-                    const float4 p = xs[base + n4 + lq], p2 = xs[base + n4 + 4 + lq];
-                    quad_test<0>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
-                    quad_test<1>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
-                    quad_test<2>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
-                    quad_test<3>(m, p.x, p.y, p.z, qx, qy, qz, r2max);
-                    quad_test<0>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
-                    quad_test<1>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
-                    quad_test<2>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
-                    quad_test<3>(m, p2.x, p2.y, p2.z, qx, qy, qz, r2max);
+        for (int base = lo; base < hi; base += 32) {
+            const int n = min(hi - base, 32);
+            const float4* __restrict__ xb = xs + base;
+            uint32_t m = 0u;
+            int n4 = 0;
+            for (; n4 < n; n4 += RV_BATCH) {
+                float4 p[RV_BATCH];
+#pragma unroll
+                for (int v = 0; v < RV_BATCH; ++v) p[v] = xb[n4 + v];
+#pragma unroll
+                for (int v = 0; v < RV_BATCH; ++v) {
+                    const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
+                    const float d2 = (dx * dx + dy * dy) + dz * dz;
+                    m = m + m + (d2 < r2max ? 1u : 0u);
                 }
-                // this lane's own range within the window: positions [ka, kb)
-                const int ka = max(lo - base, 0), kb = min(hi - base, n);
-                m &= kb > ka ? (uint32_t)(((1ull << (kb - ka)) - 1ull) << (n4 - kb)) : 0u;
-                extract(m, base, n4);
             }
-        } else {
-            for (int base = lo; base < hi; base += 32) {
-                const int n = min(hi - base, 32);
-                const float4* __restrict__ xb = xs + base;
-                uint32_t m = 0u;
-                int n4 = 0;
-                for (; n4 < n; n4 += RV_BATCH) {
-                    float4 p[RV_BATCH];
-#pragma unroll
-                    for (int v = 0; v < RV_BATCH; ++v) p[v] = xb[n4 + v];
-#pragma unroll
-                    for (int v = 0; v < RV_BATCH; ++v) {
-                        const float dx = qx - p[v].x, dy = qy - p[v].y, dz = qz - p[v].z;
-                        const float d2 = (dx * dx + dy * dy) + dz * dz;
-                        m = m + m + (d2 < r2max ? 1u : 0u);
-                    }
-                }
-                m &= ~((1u << (n4 - n)) - 1u);  // the tail past the row
-                extract(m, base, n4);
+            m &= ~((1u << (n4 - n)) - 1u);  // the tail past the row
+            while (m != 0u) {
+                const int bit = 31 - __builtin_clz(m);
+                m ^= 1u << bit;
+                f(base + (n4 - 1 - bit));
             }
         }
     });
@@ -827,7 +777,7 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     if (xi.w != D.xs[D.w_fstart[w]].w) atomicOr(&D.mixed[w], 1);
     const int cx = cell_coord(xi.x, g.lo[0], g.inv_w, g.dim[0], 0);
     const int cy = cell_coord(xi.y, g.lo[1], g.inv_w, g.dim[1], 0);
-    const int cz = cell_coord(xi.z, g.lo[2], g.inv_w, g.dim[2], 0);
+    const int cz = cell_coord(xi.z, g.lo[2], g.inv_t, g.dim[2], 0);
     const int Nf = D.Nf, cap = D.cap;
     int4* __restrict__ out = D.nbr4 + s;
     int c = 0;
@@ -943,7 +893,7 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
         finish_list();
     }
 
-    // Particles whose cell has boundary particles in its 3x3x3 block are queued with their
+    // Particles whose cell has boundary particles in its 3x3x5 block are queued with their
     // fluid sums; k_sph_bnd_density adds the boundary terms in full waves (the visit is
     // per-lane and a wave holds few such particles).  The others finish here.
     const int cell = g.cell_off + (cz * g.dim[1] + cy) * g.dim[0] + cx;
@@ -1571,10 +1521,11 @@ __global__ void k_bnd_reorder(const float* __restrict__ q, const int* __restrict
     mark_cell(skeys, s, n, btable, bcend);
 }
 
-// breach[c] = 1 when any cell of the 3x3x3 block around c (clamped to the world's box) holds
+// breach[c] = 1 when any cell of the 3x3x5 block around c (clamped to the world's box) holds
 // a boundary particle.  Every boundary candidate of a fluid particle in cell c lies in that
-// block (query ranges span at most c-1 .. c+1 per axis: the query radius is below the cell
-// width), so a zero lets the density pass skip the boundary visit.
+// block (query ranges span at most cx-1 .. cx+1, cy-1 .. cy+1 and cz-2 .. cz+2: the query
+// radius is below the cell width, twice the z cell height), so a zero lets the density pass
+// skip the boundary visit.
 __global__ void k_bnd_reach(const GridWorld* __restrict__ gw, int W, int n_cells,
                             const int2* __restrict__ btable, unsigned char* __restrict__ breach) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1589,7 +1540,7 @@ __global__ void k_bnd_reach(const GridWorld* __restrict__ gw, int W, int n_cells
     const int local = c - g.cell_off;
     const int cx = local % g.dim[0], cy = (local / g.dim[0]) % g.dim[1], cz = local / (g.dim[0] * g.dim[1]);
     unsigned char any = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int dz = -NSPH_ZROWS / 2; dz <= NSPH_ZROWS / 2; ++dz) {
         const int z = cz + dz;
         if (z < 0 || z >= g.dim[2]) continue;
         for (int dy = -1; dy <= 1; ++dy) {
@@ -1621,7 +1572,7 @@ __global__ void k_kin_keys(int n, const uint32_t* prev_keys, int2* __restrict__ 
 }
 
 // Moving boundary in sorted order (positions and velocities of state_in), its cell table, and
-// kbreach (zeroed before) over the 3x3x3 block around each occupied cell, marked by the cell's
+// kbreach (zeroed before) over the 3x3x5 block around each occupied cell, marked by the cell's
 // first particle.
 __global__ void k_kin_reorder(const float* __restrict__ q, const float* __restrict__ qd, const int* __restrict__ kid,
                               const int* __restrict__ kworld, const GridWorld* __restrict__ gw,
@@ -1640,7 +1591,7 @@ __global__ void k_kin_reorder(const float* __restrict__ q, const float* __restri
     const GridWorld g = gw[kworld[k]];
     const int local = cell - g.cell_off;
     const int cx = local % g.dim[0], cy = (local / g.dim[0]) % g.dim[1], cz = local / (g.dim[0] * g.dim[1]);
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int dz = -NSPH_ZROWS / 2; dz <= NSPH_ZROWS / 2; ++dz) {
         const int z = cz + dz;
         if (z < 0 || z >= g.dim[2]) continue;
         for (int dy = -1; dy <= 1; ++dy) {
@@ -2041,13 +1992,15 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         for (int a = 0; a < 3; ++a) {
             const double lo = prm->h_domain_lo[3 * w + a], hi = prm->h_domain_hi[3 * w + a];
             NSPH_CHECK(hi > lo, NSPH_ERR_INVALID, "nsph_solver_create: empty domain in world %d", w);
-            double d = std::ceil((hi - lo) / cw) + 1.0;
+            const double wa = a < 2 ? cw : 0.5 * cw;  // z cells are half as high
+            double d = std::ceil((hi - lo) / wa) + 1.0;
             if (d < 3.0) d = 3.0;
             NSPH_CHECK(d < 1.0e6, NSPH_ERR_INVALID, "nsph_solver_create: domain too large for the cell size in world %d", w);
             g.lo[a] = (float)lo;
             g.dim[a] = (int)d;
         }
         g.inv_w = (float)(1.0 / cw);
+        g.inv_t = 2.0f * g.inv_w;  // z
         g.wrap = 0;
         g.cell_off = (int)off;
         off += (size_t)g.dim[0] * g.dim[1] * g.dim[2];

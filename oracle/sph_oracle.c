@@ -46,7 +46,8 @@ typedef struct {
 
 typedef struct {
     float lo[3];
-    float inv_w;
+    float inv_w;  /* inverse cell width along x */
+    float inv_t;  /* inverse cell height along z: 2 inv_w (half-height cells) */
     int dim[3];
     int cell_off;
 } orc_gw;
@@ -79,8 +80,10 @@ static int cell_raw(float x, float lo, float inv_w) {
 
 static int clampi(int c, int dim) { return c < 0 ? 0 : (c >= dim ? dim - 1 : c); }
 
-/* Cells are split into ORC_XSUB slabs along x for sorting: the solver's order is
- * (world, cz, cy, cx, slab, id) with slab coordinate clamp(floor(XSUB (x - lo) inv_w)). */
+/* Cells are w wide along x and y and w / 2 high along z, and split into ORC_XSUB slabs along
+ * x for sorting: the solver's order is (world, cz, cy, cx, slab, id) with slab coordinate
+ * clamp(floor(XSUB (x - lo) inv_w)), cy = clamp(floor((y - lo) inv_w)) and
+ * cz = clamp(floor((z - lo) inv_t)). */
 #define ORC_XSUB_BITS 2
 #define ORC_XSUB (1 << ORC_XSUB_BITS)
 
@@ -91,7 +94,7 @@ static uint32_t key_of(const orc_gw* g, float x, float y, float z) {
     const int top = g->dim[0] * ORC_XSUB - 1;
     gx = gx < 0 ? 0 : (gx > top ? top : gx);
     const int cy = clampi(cell_raw(y, g->lo[1], g->inv_w), g->dim[1]);
-    const int cz = clampi(cell_raw(z, g->lo[2], g->inv_w), g->dim[2]);
+    const int cz = clampi(cell_raw(z, g->lo[2], g->inv_t), g->dim[2]);
     return (uint32_t)(g->cell_off + (cz * g->dim[1] + cy) * g->dim[0]) * ORC_XSUB + (uint32_t)gx;
 }
 
@@ -122,12 +125,13 @@ static size_t make_grids(const orc_problem* pb, orc_gw* gws) {
         const double cw = H * (1.0 + pb->skin) * (1.0 + 1.0 / 128.0);  /* the search radius */
         for (int a = 0; a < 3; ++a) {
             const double lo = pb->domain_lo[3 * w + a], hi = pb->domain_hi[3 * w + a];
-            double d = ceil((hi - lo) / cw) + 1.0;
+            double d = ceil((hi - lo) / (a < 2 ? cw : 0.5 * cw)) + 1.0;
             if (d < 3.0) d = 3.0;
             gws[w].lo[a] = (float)lo;
             gws[w].dim[a] = (int)d;
         }
         gws[w].inv_w = (float)(1.0 / cw);
+        gws[w].inv_t = 2.0f * gws[w].inv_w;
         gws[w].cell_off = (int)off;
         off += (size_t)gws[w].dim[0] * gws[w].dim[1] * gws[w].dim[2];
     }
@@ -157,7 +161,7 @@ static int scan_cells(const orc_gw* g, const int* tab, const float* pts, int s, 
     int x0, nx, y0, ny, z0, nz;
     axis_range(qx, rp, g->lo[0], g->inv_w, g->dim[0], &x0, &nx);
     axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
-    axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
+    axis_range(qz, rp, g->lo[2], g->inv_t, g->dim[2], &z0, &nz);
     int c = 0;
     for (int iz = 0; iz < nz; ++iz)
         for (int iy = 0; iy < ny; ++iy) {
@@ -382,7 +386,7 @@ static int orc_layout_build(const orc_problem* pb, const float* q_in, const floa
         int x0, nx, y0, ny, z0, nz;
         axis_range(qx, rp, g->lo[0], g->inv_w, g->dim[0], &x0, &nx);
         axis_range(qy, rp, g->lo[1], g->inv_w, g->dim[1], &y0, &ny);
-        axis_range(qz, rp, g->lo[2], g->inv_w, g->dim[2], &z0, &nz);
+        axis_range(qz, rp, g->lo[2], g->inv_t, g->dim[2], &z0, &nz);
         L->bstart[s] = (int)bn;
         for (int grp = 0; grp < 2; ++grp) {  /* static boundary, then moving */
             const int* tab = grp == 0 ? bt : kt;
