@@ -557,12 +557,42 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
         const float e0 = c == 0 ? -INFINITY : (float)c, e1 = c == dim - 1 ? INFINITY : (float)(c + 1);
         return fmaxf(fmaxf(e0 - v, v - e1), 0.0f);
     };
-    // clip row k to the sphere and load its table entries; false when the row is skipped
+    // squared distances (in cell widths) to the three y rows and the five z rows, once per
+    // query, held in registers that rotate as the rows advance (y fastest): (gy0, gz0) are
+    // always those of the next row to be loaded
+    float gy0, gy1, gy2, gz0, gz1, gz2, gz3, gz4;
+    {
+        float t;
+        t = gap(fy, y0, g.dim[1]), gy0 = t * t;
+        t = gap(fy, y0 + 1, g.dim[1]), gy1 = t * t;
+        t = gap(fy, y0 + 2, g.dim[1]), gy2 = t * t;
+        t = 0.5f * gap(fz, z0, g.dim[2]), gz0 = t * t;
+        t = 0.5f * gap(fz, z0 + 1, g.dim[2]), gz1 = t * t;
+        t = 0.5f * gap(fz, z0 + 2, g.dim[2]), gz2 = t * t;
+        t = 0.5f * gap(fz, z0 + 3, g.dim[2]), gz3 = t * t;
+        t = 0.5f * gap(fz, z0 + 4, g.dim[2]), gz4 = t * t;
+    }
+    static_assert(NSPH_ZROWS == 5, "five z rows");
+    auto advance = [&](int k) {  // row k was loaded: move to row k + 1
+        const float t = gy0;
+        gy0 = gy1;
+        gy1 = gy2;
+        gy2 = t;
+        if (k % 3 == 2) {
+            gz0 = gz1;
+            gz1 = gz2;
+            gz2 = gz3;
+            gz3 = gz4;
+        }
+    };
+    const int row0 = g.cell_off + (z0 * g.dim[1] + y0) * g.dim[0] + x0;
+    // clip row k to the sphere and load its table entries (the row's three cells: cells past a
+    // row of fewer than three are loaded too, from the next row or the table's two padding
+    // entries, and ignored); false when the row is skipped
     auto load_row = [&](int k, int2 c[3], int* fa, int* fb, int* row) {
         const int iz = k / 3, iy = k - 3 * (k / 3);
         if (!(iz < nz && iy < ny)) return false;
-        const float gy = gap(fy, y0 + iy, g.dim[1]), gz = 0.5f * gap(fz, z0 + iz, g.dim[2]);
-        const float h2 = rc2 - gy * gy - gz * gz;
+        const float h2 = (rc2 - gy0) - gz0;
         if (!(h2 > 0.0f)) return false;
         // the 1-ulp hardware root: the 1/64-cell pad of rc covers its error
         const float hx = __builtin_amdgcn_sqrtf(h2);
@@ -573,20 +603,23 @@ __device__ inline void rows_for_each(const GridWorld& g, const int2* __restrict_
         *fa = min(max((int)a, xa), xb);
         *fb = max(min((int)b, xb), xa);
         if (*fa > *fb) return false;
-        *row = g.cell_off + ((z0 + iz) * g.dim[1] + (y0 + iy)) * g.dim[0] + x0;
+        *row = row0 + (iz * g.dim[1] + iy) * g.dim[0];
+        const int2* __restrict__ tp = table + *row;
 #pragma unroll
-        for (int ix = 0; ix < 3; ++ix) c[ix] = table[*row + min(ix, nx - 1)];
+        for (int ix = 0; ix < 3; ++ix) c[ix] = tp[ix];
         return true;
     };
     constexpr int K = 3 * NSPH_ZROWS;
     int2 c[3];
     int fa = 0, fb = 0, row = 0;
     bool ok = load_row(0, c, &fa, &fb, &row);
+    advance(0);
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
         int2 cn[3];
         int fan = 0, fbn = 0, rown = 0;
         const bool okn = k < K - 1 && load_row(k + 1, cn, &fan, &fbn, &rown);
+        advance(k + 1);
         // every lane calls scan (an empty range for a skipped row), so a scan may exchange
         // data between lanes
         int2 rr = make_int2(0, 0);
@@ -2043,12 +2076,13 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(radix_reserve(S->rs, nmax > 0 ? nmax : 1));
     NSPH_TRY(buf_ensure(S->keys, (size_t)(nmax + 1) * 4));
     NSPH_TRY(buf_ensure(S->vals, (size_t)(nmax + 1) * 4));
-    NSPH_TRY(buf_ensure(S->btable, (off + 1) * sizeof(int2)));
-    NSPH_TRY(buf_ensure(S->ftable, (off + 1) * sizeof(int2)));
+    // two entries past the last cell: a row's three table entries are loaded together
+    NSPH_TRY(buf_ensure(S->btable, (off + 2) * sizeof(int2)));
+    NSPH_TRY(buf_ensure(S->ftable, (off + 2) * sizeof(int2)));
     NSPH_TRY(buf_ensure(S->bcend, (off + 1) * sizeof(int)));
     NSPH_TRY(buf_ensure(S->fcend, (off + 1) * sizeof(int)));
-    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->btable), off + 1, make_int2(0, 0));
-    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ftable), off + 1, make_int2(0, 0));
+    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->btable), off + 2, make_int2(0, 0));
+    hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ftable), off + 2, make_int2(0, 0));
     NSPH_TRY(buf_ensure(S->bxs, (size_t)(Nb + 1) * 16));
     NSPH_TRY(buf_ensure(S->flags, 16));
 
@@ -2089,11 +2123,11 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
         NSPH_TRY(buf_ensure(S->kvals, (size_t)(Nk + 1) * 4));
         NSPH_TRY(buf_ensure(S->kxs, (size_t)(Nk + 1) * 16));
         NSPH_TRY(buf_ensure(S->kvs, (size_t)(Nk + 1) * 16));
-        NSPH_TRY(buf_ensure(S->ktable, (off + 1) * sizeof(int2)));
+        NSPH_TRY(buf_ensure(S->ktable, (off + 2) * sizeof(int2)));
         NSPH_TRY(buf_ensure(S->kcend, (off + 1) * sizeof(int)));
         NSPH_TRY(buf_ensure(S->kbreach, (off + 4) & ~(size_t)3));
         NSPH_TRY(buf_ensure(S->kvb, ((size_t)Nf + 1) * 4));
-        hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ktable), off + 1, make_int2(0, 0));
+        hipLaunchKernelGGL(k_fill_int2, dim3(2048), dim3(256), 0, st, bp<int2>(S->ktable), off + 2, make_int2(0, 0));
         hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, st, bp<int>(S->kkeys), (size_t)(Nk + 1), 0);
         S->kskeys = bp<uint32_t>(S->kkeys);
         uint32_t* unused = nullptr;
