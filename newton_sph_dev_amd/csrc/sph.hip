@@ -1021,7 +1021,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
     float ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XV, 4>(
+    for_each_nbr<XV, 8>(
         D, s, [&](int j) { return XV{D.xs[j], D.vs[j]}; },
         [&](const XV& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
@@ -1059,7 +1059,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     const float p0 = prev ? 0.5f * D.pws[s] : 0.f;
     const float pri = p0 / (rho * rho);
     float sum = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
-    for_each_nbr<XA, 4>(
+    for_each_nbr<XA, 8>(
         D, s, [&](int j) { return XA{D.xs[j], D.vadv[j]}; },
         [&](const XA& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
@@ -1170,7 +1170,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
         const float4 xi = D.xs[s];
         const float4 ai = D.ap[s];
         float sum = 0.f;
-        for_each_nbr<XA, 4>(
+        for_each_nbr<XA, 8>(
             D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
             [&](const XA& o, int) {
                 const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
