@@ -14,8 +14,8 @@
 //   xpr[s]  float4 (x, y, z, p/rho^2)   (positions again, so that the pressure-acceleration
 //                                     gather of a world of one fluid mass loads 16 bytes)
 //   gb[s]   float4 (sum_b gradW_ib, boundary count)
-//   avisc[s], ap[s]  float4      vadv[s] float4 (v_adv, pr0), pr0 = 0.5 p_prev / rho^2 under the
-//                                previous-pressure warm start, else 0
+//   avisc[s] float4 (a_visc, velocity divergence term)   ap[s] float4 (a^p, 0)
+//   vadv[s]  float4 (v_adv, pr0), pr0 = p0 / rho^2 of the warm start (0 under the zero one)
 //   nbr4[q*Nf + s] int4  fluid-fluid neighbour list, slots 4q..4q+3 of particle s (lanes
 //                        of a wave read 64 consecutive quads); ascending in s
 //   fp[s]   float4 (SPH force, p) at the step's end;  inv[k] sorted position of fluid index k
@@ -1020,7 +1020,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
     const PhysWorld P = D.phys[w];
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    float ax = 0.f, ay = 0.f, az = 0.f, dv = 0.f;
     for_each_nbr<XV, 8>(
         D, s, [&](int j) { return XV{D.xs[j], D.vs[j]}; },
         [&](const XV& o, int) {
@@ -1032,9 +1032,15 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
             ax += c * (gf * dx);
             ay += c * (gf * dy);
             az += c * (gf * dz);
+            dv += o.x.w * (gf * vx);
         });
     const float avx = P.visc10 * ax, avy = P.visc10 * ay, avz = P.visc10 * az;
-    D.avisc[s] = make_float4(avx, avy, avz, 0.f);
+    // avisc.w: sum_j m_j (v_i - v_j) . gradW_ij + psi (v_i . Gb_i - sum_b v_b . gradW_ib), the
+    // velocity divergence term of the source with the velocities at the step's start (an input
+    // of the learned warm start, which runs before the source pass)
+    const float4 gb = D.gb[s];
+    dv += P.psi * (dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z) - (D.kvb ? D.kvb[s] : 0.f));
+    D.avisc[s] = make_float4(avx, avy, avz, dv);
     const float pr0 = warm == NSPH_WARM_PREV ? (0.5f * D.pws[s]) / (vi.w * vi.w) : 0.0f;
     D.vadv[s] = make_float4(vi.x + dt * (P.g[0] + avx), vi.y + dt * (P.g[1] + avy),
                             vi.z + dt * (P.g[2] + avz), pr0);
@@ -1042,12 +1048,12 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
 
 // Source term and warm start:
 //   s_i = (rho0 - rho_i) - dt (sum_j m_j (v_adv_i - v_adv_j) . gradW_ij + psi v_adv_i . Gb_i)
-//   p_i = 0 | 0.5 p_prev | external;  pr_i = p_i / rho_i^2
-// Source term fused with the first pressure acceleration (zero and previous-step warm starts,
-// whose p0 every particle can form from its own inputs): one pass over the list gives s_i
-// and a^p_i of iteration 0 from pr_j = p0_j / rho_j^2 (vadv.w, formed by k_sph_advect with
-// the same operations as pri here), the same operations, operands and order as
-// k_sph_source followed by k_sph_pacc.
+//   p_i = 0 | 0.5 p_prev | learned;  pr_i = p_i / rho_i^2
+// Source term fused with the first pressure acceleration: one pass over the list gives s_i
+// and a^p_i of iteration 0 from pr_j = p0_j / rho_j^2 (vadv.w, formed with the same operations
+// as pri here by k_sph_advect for the zero and previous-pressure warm starts, or by
+// k_sph_mlp_warm, which runs between the two passes, for the learned one), the same
+// operations, operands and order as a source pass followed by k_sph_pacc.
 __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
     int w, s;
     if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
@@ -1055,8 +1061,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     const float4 xi = D.xs[s];
     const float4 vi = D.vadv[s];
     const float rho = D.rho[s];
-    const bool prev = warm == NSPH_WARM_PREV;
-    const float p0 = prev ? 0.5f * D.pws[s] : 0.f;
+    const float p0 = warm == NSPH_WARM_PREV ? 0.5f * D.pws[s] : (warm == NSPH_WARM_MLP ? D.p0[s] : 0.f);
     const float pri = p0 / (rho * rho);
     float sum = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
     for_each_nbr<XA, 8>(
@@ -1081,33 +1086,6 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pri);
     const float cb = P.psi * pri;
     D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
-}
-
-__global__ __launch_bounds__(CHUNK) void k_sph_source(SphDev D, float dt, int warm) {
-    int w, s;
-    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
-    const PhysWorld P = D.phys[w];
-    const float4 xi = D.xs[s];
-    const float4 vi = D.vadv[s];
-    float sum = 0.f;
-    for_each_nbr<XA, 4>(
-        D, s, [&](int j) { return XA{D.xs[j], D.vadv[j]}; },
-        [&](const XA& o, int) {
-            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
-            const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const float gf = kern_gf(P, r2);
-            sum += o.x.w * dot3(vi.x - o.a.x, vi.y - o.a.y, vi.z - o.a.z, gf * dx, gf * dy, gf * dz);
-        });
-    const float4 gb = D.gb[s];
-    sum += P.psi * (dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z) - (D.kvb ? D.kvb[s] : 0.f));
-    const float rho = D.rho[s];
-    D.src[s] = (P.rho0 - rho) - dt * sum;
-    float p0 = 0.f;
-    if (warm == NSPH_WARM_PREV) p0 = 0.5f * D.pws[s];
-    else if (warm == NSPH_WARM_MLP) p0 = D.pws[s];
-    D.p[s] = p0;
-    D.p0[s] = p0;
-    D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
 }
 
 // Pressure acceleration  a^p_i = -( sum_j m_j (pr_i + pr_j) gradW_ij + (psi pr_i) Gb_i ).
@@ -1386,8 +1364,9 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
 //
 // Features of sorted fluid particle s (all from the prologue's buffers):
 //   f0 = 100 (rho - rho0) / rho0                  density error, %
-//   f1 = 100 ((rho0 - rho) - s_i) / rho0          = 100 dt (sum_j m_j v_ij . gradW_ij) / rho0,
-//                                                   density change from the velocity divergence, %
+//   f1 = 100 dt (sum_j m_j v_ij . gradW_ij + psi (v_i . Gb_i - sum_b v_b . gradW_ib)) / rho0
+//        with the velocities at the step's start (k_sph_advect, avisc.w): the density change
+//        from the velocity divergence, %
 //   f2 = (fluid + boundary neighbours) / 32
 //   f3 = p_prev / p_unit                          previous pressure
 //   f4 = 100 |v_adv| dt / H                       displacement per step, % of H
@@ -1398,12 +1377,12 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
 // all-zero network reproduces the 0.5 p_prev warm start exactly.
 __device__ inline float mlp_features(const SphDev& D, const PhysWorld& P, int s, float dt,
                                      float* f) {
-    const float rho = D.rho[s], src = D.src[s], a = D.aii[s];
+    const float rho = D.rho[s], a = D.aii[s];
     const float2 ax = D.aux[s];
     const float4 v = D.vadv[s];
     const float punit = a < 0.0f ? (P.rho0 * 0.01f) / (-a) : 0.0f;
     f[0] = (rho - P.rho0) / P.rho0 * 100.0f;
-    f[1] = ((P.rho0 - rho) - src) / P.rho0 * 100.0f;
+    f[1] = (dt * D.avisc[s].w) / P.rho0 * 100.0f;  // dt x velocity divergence term, in % of rho0
     f[2] = ((float)D.ncnt[s] + ax.x) * (1.0f / 32.0f);
     f[3] = punit > 0.0f ? D.pws[s] / punit : 0.0f;
     f[4] = sqrtf(dot3(v.x, v.y, v.z, v.x, v.y, v.z)) * dt * P.invH * 100.0f;
@@ -1469,10 +1448,8 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
         const float y = bulk ? fminf(fmaxf(ys[t], -MLP_Y_MAX), MLP_Y_MAX) : 0.0f;
         const float p0 = fmaxf(0.5f * D.pws[s] + y * punit, 0.0f);
         const float rho = D.rho[s];
-        const float4 xi = D.xs[s];
-        D.p[s] = p0;
         D.p0[s] = p0;
-        D.xpr[s] = make_float4(xi.x, xi.y, xi.z, p0 / (rho * rho));
+        reinterpret_cast<float*>(D.vadv + s)[3] = p0 / (rho * rho);  // pr_j of the source pass
     }
 }
 
@@ -2385,12 +2362,9 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     });
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
-        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source, gg, gb, 0, st, D, dt, S->warm));
-    else
-        TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source_pacc, gg, gb, 0, st, D, dt, S->warm));
-    if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_MLP, hipLaunchKernelGGL(k_sph_mlp_warm, dim3(C), dim3(CHUNK), 0, st, D,
                                          nsph_mlp_device_view(S->mlp), dt));
+    TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source_pacc, gg, gb, 0, st, D, dt, S->warm));
     NSPH_HIP(hipGetLastError());
     S->last_dt = dt;
     return NSPH_OK;
@@ -2407,9 +2381,8 @@ static int enqueue_iterations(nsph_solver* S, hipStream_t st, float dt, int it0,
     int launched = it0;
     if (S->Nf > 0) {
         for (int it = it0; it < it1; ++it) {
-            // iteration 0's pressure acceleration comes from the fused source pass unless the
-            // MLP warm start set p0 after it
-            if (it > 0 || S->warm == NSPH_WARM_MLP)
+            // iteration 0's pressure acceleration comes from the fused source pass
+            if (it > 0)
                 TIMED(KT_PACC, hipLaunchKernelGGL(k_sph_pacc, dim3(C * D.nsub), dim3(S->gblock), 0, st, D));
             TIMED(KT_PUPDATE, hipLaunchKernelGGL(k_sph_pupdate, dim3(C), dim3(CHUNK), 0, st, D, dt2,
                                                  (float)S->omega));
