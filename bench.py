@@ -71,13 +71,18 @@ def sort_passes(params) -> int:
     return min((bits + 7) // 8, (bits + 8) // 9)
 
 
-def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -> dict[str, float]:
+def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True, iters: float = 2.0,
+                      solve_passes: int = 0) -> dict[str, float]:
     """Compulsory bytes per fluid particle and launch, per kernel class (nbar = mean
     fluid-fluid list length; 4 B per list entry + 4 B count).  `passes` radix passes of
     20 B each; the source pass also writes iteration 0's pressure acceleration (16 B) unless
-    the MLP warm start runs between them."""
+    the MLP warm start runs between them.  The whole-world solve runs, in one launch, the
+    source pass (after advection, which runs as its own launch with the MLP warm start),
+    `iters` pressure updates, iters - 1 pressure accelerations and the forces; after
+    `solve_passes` per-pass iterations it runs only the remaining iterations (an upper bound:
+    worlds that stopped earlier skip it)."""
     L = 4.0 * nbar + 4.0
-    return {
+    t = {
         "build": 28 + passes * 20 + 92 + 16,
         "neighbors_density": 16 + 4 + 4 + 16 + L,
         # xs, vs, pws -> avisc, vadv (v_adv, p0 / rho^2)
@@ -96,6 +101,13 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True) -
         # inv, fid, fworld, fp, xs, vs, inv_mass -> q, qd, f, pprev (particle order)
         "integrate": 4 + 4 + 4 + 16 + 16 + 16 + 4 + 40,
     }
+    if solve_passes > 0:
+        rest = max(iters - solve_passes, 0.0)
+        t["solve_worlds"] = rest * (t["pressure_update"] + t["pressure_acc"])
+    else:
+        t["solve_worlds"] = ((t["advect"] if fused_source else 0.0) + t["source"] + iters * t["pressure_update"]
+                             + max(iters - 1.0, 0.0) * t["pressure_acc"] + t["final"])
+    return t
 
 
 def host_cpu() -> tuple[int, str]:
@@ -290,17 +302,32 @@ def extra_configs(log) -> dict:
     b, H = scenes.build_batched(bcfg)
     m = b.finalize("cuda:0")
     del b
-    sol = SolverSPH(m, H, warm_start="prev")
-    a, c = m.state(), m.state()
-    _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 3)
-    ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 10)
-    out["C3_batched_4096"] = {
-        "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
-                   "boundary_particles": sol.boundary_count, "steps": 10, "warmup": 3, "dt": scenes.DT,
-                   "warm_start": "prev", "launch": "eager, one synchronisation per step"},
-        "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
-        "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
-    del sol, a, c, m
+    # the whole-world schedule (each workgroup iterates whole worlds to their own stop test in
+    # one persistent launch, no host read inside the step), and the per-pass one beside it
+    c3 = None
+    for solve, block in (("worlds", 512), ("passes", 256)):
+        sol = SolverSPH(m, H, warm_start="prev", solve=solve, solve_block=block)
+        a, c = m.state(), m.state()
+        _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 3)
+        ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 10)
+        if c3 is None:
+            c3 = {
+                "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
+                           "boundary_particles": sol.boundary_count, "steps": 10, "warmup": 3, "dt": scenes.DT,
+                           "warm_start": "prev",
+                           "launch": f"eager; whole-world solve: one persistent launch of {block}-thread "
+                                     "workgroups taking whole worlds, largest first, each iterated to its own "
+                                     "stop test on the device; one report read per step"},
+                "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
+                "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
+        else:
+            c3["per_pass_schedule"] = {"ms_per_step": ms, "latency_ms": _pct(lat),
+                                       "launch": "eager, one launch per pass over all worlds, one host read of "
+                                                 "the still-iterating count per step"}
+        del sol, a, c
+        free()
+    out["C3_batched_4096"] = c3
+    del m
     free()
 
     # C5: 1024 batched worlds (uniform 4096..65536), learned warm start, HIP graph replays
@@ -309,22 +336,38 @@ def extra_configs(log) -> dict:
     b, H = scenes.build_batched(bcfg)
     m = b.finalize("cuda:0")
     del b
-    sol = SolverSPH(m, H)
-    sol.set_mlp(WarmStartMLP.load(os.path.join(data, "warmstart_mlp.npy")))
-    a, c = m.state(), m.state()
-    _, _, its0, (a, c) = _timed_steps(sol, a, c, scenes.DT, 6)
-    slots = max(its0[-3:]) + 1  # the first steps from rest need more; a replay finishes eagerly if short
-    sol.use_graph, sol.graph_slots = True, slots
-    _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 2)  # captures both directions
-    ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 20)
-    out["C5_batched_1024_mlp_graph"] = {
-        "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
-                   "boundary_particles": sol.boundary_count, "steps": 20, "dt": scenes.DT, "warm_start": "mlp",
-                   "launch": f"HIP graph replay, {slots} captured Jacobi slots; one host read of the "
-                             "still-iterating count after the slots per step"},
-        "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
-        "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
-    del sol, a, c, m
+    mlp = WarmStartMLP.load(os.path.join(data, "warmstart_mlp.npy"))
+    c5 = None
+    # graph replays of the whole-world schedule after 2 per-pass iterations (no host read inside
+    # the step), and of the per-pass schedule with captured iteration slots beside it
+    for solve in ("worlds", "passes"):
+        sol = SolverSPH(m, H, solve=solve, solve_block=512, solve_passes=2)
+        sol.set_mlp(mlp)
+        a, c = m.state(), m.state()
+        _, _, its0, (a, c) = _timed_steps(sol, a, c, scenes.DT, 6)
+        slots = max(its0[-3:]) + 1  # the first steps from rest need more; a replay finishes eagerly if short
+        sol.use_graph, sol.graph_slots = True, slots
+        _, _, _, (a, c) = _timed_steps(sol, a, c, scenes.DT, 2)  # captures both directions
+        ms, lat, its, _ = _timed_steps(sol, a, c, scenes.DT, 20)
+        if c5 is None:
+            c5 = {
+                "config": {"workload": bcfg.name, "worlds": bcfg.n_worlds, "fluid_particles": sol.fluid_count,
+                           "boundary_particles": sol.boundary_count, "steps": 20, "dt": scenes.DT,
+                           "warm_start": "mlp",
+                           "launch": "HIP graph replay; 2 Jacobi iterations as launches over all worlds, then "
+                                     "one persistent launch of 512-thread workgroups iterating each world still "
+                                     "iterating to its own stop test; no host read inside the step, one report "
+                                     "read per step"},
+                "value": sol.fluid_count * 1e3 / ms, "unit": "particle-updates/s", "ms_per_step": ms,
+                "latency_ms": _pct(lat), "mean_iterations": float(np.mean(its))}
+        else:
+            c5["per_pass_schedule"] = {"ms_per_step": ms, "latency_ms": _pct(lat),
+                                       "launch": f"HIP graph replay, {slots} captured Jacobi slots; one host "
+                                                 "read of the still-iterating count after the slots per step"}
+        del sol, a, c
+        free()
+    out["C5_batched_1024_mlp_graph"] = c5
+    del m
     free()
     return out
 
@@ -345,6 +388,11 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay captured HIP graphs")
     ap.add_argument("--graph-slots", type=int, default=0,
                     help="Jacobi iterations captured in the graph (0: warm-up maximum + 1)")
+    ap.add_argument("--solve", default="passes", choices=("passes", "worlds"),
+                    help="pressure-solve schedule: launches per pass, or the persistent whole-world launch")
+    ap.add_argument("--solve-block", type=int, default=512, help="workgroup size of the whole-world launch")
+    ap.add_argument("--solve-passes", type=int, default=0,
+                    help="Jacobi iterations run per pass over all worlds before the whole-world launch")
     ap.add_argument("--autotune", action="store_true", help="tune the launch configuration first")
     ap.add_argument("--mlp", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                   "newton_sph_dev_amd", "data", "warmstart_mlp.npy"),
@@ -376,7 +424,8 @@ def main():
         model = scenes.build_dam_break(cfg).finalize("cuda:0")
         support = cfg.support
     solver = SolverSPH(model, support, warm_start="zero" if args.warm_start == "mlp" else args.warm_start,
-                       check_every=args.check_every, skin=args.skin)
+                       check_every=args.check_every, skin=args.skin, solve=args.solve,
+                       solve_block=args.solve_block, solve_passes=args.solve_passes)
     if args.warm_start == "mlp":
         from newton_sph_dev_amd.learned import WarmStartMLP
 
@@ -423,8 +472,10 @@ def main():
     graph_mode = solver.use_graph
     solver.use_graph = False
     N.call("nsph_solver_set_timing", solver._handle, -1)
+    probe_iters = []
     for _ in range(PROBE_STEPS):
         solver.step(s0, s1, None, None, dt)
+        probe_iters.append(solver.last_report["sum_iterations"] / max(solver.W, 1))
         s0, s1 = s1, s0
     N.call("nsph_solver_timing", solver._handle, ms, cnt)
     solver.use_graph = graph_mode
@@ -432,7 +483,9 @@ def main():
     counts = solver.fields()["neighbor_count"]
     nf = solver.fluid_count
     nbar = float(counts.sum().item()) / max(nf, 1)
-    ab = algorithmic_bytes(nbar, sort_passes(solver.params), fused_source=args.warm_start != "mlp")
+    ab = algorithmic_bytes(nbar, sort_passes(solver.params), fused_source=args.warm_start != "mlp",
+                           iters=float(np.mean(probe_iters)),
+                           solve_passes=solver.solve_passes if solver.solve == "worlds" else 0)
     dom = max((k for k in names if ab.get(k, 0.0) > 0), key=lambda k: per_all[k][0])
     # timed region: events only around the dominant class
     N.call("nsph_solver_set_timing", solver._handle, 1 << names.index(dom))
@@ -516,7 +569,9 @@ def main():
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "launch": tuned or {"gather_block": solver.gather_block, "graph": solver.use_graph,
                             "graph_slots": solver.graph_slots if solver.use_graph else None,
-                            "check_every": solver.check_every},
+                            "check_every": solver.check_every, "solve": solver.solve,
+                            "solve_block": solver.solve_block if solver.solve == "worlds" else None,
+                            "solve_passes": solver.solve_passes if solver.solve == "worlds" else None},
         "kernels": kern,
     }
     if not args.no_cpu_baseline:

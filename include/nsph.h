@@ -327,6 +327,21 @@ int nsph_solver_set_launch(nsph_solver* s, int gather_block);
  * default), 4, 8, 2; anything else fails with NSPH_ERR_INVALID.  Scheduling only: results are
  * identical for every variant.  Set by the autotuner. */
 int nsph_solver_set_search(nsph_solver* s, int lds_quads);
+/* Schedule of the pressure solve (replaces the per-pass launches behind SolverBase.step,
+ * solver.py:301-316; batched worlds as model.py:170-187 groups them).
+ * NSPH_SOLVE_PASSES (the default): one launch per pass over all worlds, the host reading the
+ * still-iterating count once per step.  NSPH_SOLVE_WORLDS: one persistent launch, each
+ * workgroup (`block` = 256, 512 or 1024 threads) taking whole worlds from a device queue,
+ * largest first, iterates every world to its own stop test.  With pass_iters = 0 it runs
+ * advection, source, the Jacobi iterations and the forces (with the learned warm start,
+ * advection and the network run as their own launches over all worlds before it); with
+ * pass_iters = k > 0 the first k iterations run as per-pass launches over all worlds and the
+ * persistent launch only finishes the worlds still iterating.  Either way no host read, and a
+ * captured step holds no iteration slots beyond the k.
+ * Scheduling only: results are identical for both.  Set by the autotuner; captured graphs
+ * must be captured again after a change (nsph_solver_graph_step fails otherwise). */
+enum nsph_solve_mode { NSPH_SOLVE_PASSES = 0, NSPH_SOLVE_WORLDS = 1 };
+int nsph_solver_set_solve(nsph_solver* s, int mode, int block, int pass_iters);
 /* Per-kernel-class GPU timing with HIP events recorded on the context stream around each
  * launch group of the classes whose bit is set in `enable` (bit k = class k; -1 = all; 0 = off,
  * the default). Changing it resets the totals. nsph_solver_timing
@@ -338,7 +353,8 @@ int nsph_solver_timing(nsph_solver* s, double* h_ms, int* h_launches);
 int nsph_kernel_count(void);
 const char* nsph_kernel_name(int k);
 /* Jacobi iterations launched by the last step (>= the max over worlds of iterations
- * applied when check_every > 1 or 0; the extra ones exit at once). */
+ * applied when check_every > 1 or 0; the extra ones exit at once); -1 after a whole-world
+ * solve, which iterates inside one launch. */
 int nsph_solver_iterations_launched(const nsph_solver* s, int* n);
 int nsph_solver_destroy(nsph_solver* s);
 

@@ -147,6 +147,7 @@ SIGNATURES: dict[str, list] = {
     "nsph_solver_set_timing": [_vp, _i],
     "nsph_solver_set_launch": [_vp, _i],
     "nsph_solver_set_search": [_vp, _i],
+    "nsph_solver_set_solve": [_vp, _i, _i, _i],
     "nsph_solver_timing": [_vp, _pd, _pi],
     "nsph_kernel_count": [],
     "nsph_solver_iterations_launched": [_vp, _pi],

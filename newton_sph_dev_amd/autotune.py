@@ -10,9 +10,15 @@ Tunables (none of them changes a single bit of the results, which the GPU tests 
                  "graph": the step replayed from HIP graphs with `graph_slots` captured Jacobi
                  iterations (one host read after the slots; eager iterations finish the solve
                  if the slots were too few)
+  solve          the pressure solve's schedule (batches of at least MIN_WORLDS_SOLVE worlds):
+                 "passes" (one launch per pass over all worlds) or "worlds" (one persistent
+                 launch whose workgroups of `solve_block` threads take whole worlds from a
+                 queue, largest first, after `solve_passes` per-pass Jacobi iterations), eager
+                 or captured: the work partition between worlds of uneven size
 The tuner times each candidate over a few steps of the caller's own solver and state
-(ping-pong, so the physics keeps advancing) and keeps the fastest, in two rounds: the search
-variant first (eager, the current gather block), then gather block x mode with that variant.
+(ping-pong, so the physics keeps advancing) and keeps the fastest, in three rounds: the search
+variant first (eager, the current gather block), then gather block x mode with that variant,
+then the solve schedules with the best of those.
 Results are persisted as JSON keyed by GPU architecture, fluid-count bucket and world count.
 """
 
@@ -66,8 +72,31 @@ def candidates(mean_iterations: float, search=None) -> list[dict]:
     return out
 
 
+# a whole-world schedule needs enough worlds to fill the GPU's workgroup slots
+MIN_WORLDS_SOLVE = 64
+
+
+def solve_candidates(base: dict, solver) -> list[dict]:
+    """Whole-world schedules on top of `base` (the best per-pass configuration)."""
+    if solver.W < MIN_WORLDS_SOLVE:
+        return []
+    out = []
+    for block in (256, 512, 1024):
+        for k in sorted({0, max(solver.min_iters, 1)}):
+            for mode in ("eager", "graph"):
+                c = {"gather_block": base["gather_block"], "search": base.get("search", _variants()[0]),
+                     "mode": mode, "solve": "worlds", "solve_block": block, "solve_passes": k}
+                if mode == "graph":
+                    c["graph_slots"] = None
+                else:
+                    c["check_every"] = 1
+                out.append(c)
+    return out
+
+
 def apply(solver, cfg: dict) -> None:
     solver.set_launch(cfg["gather_block"], int(cfg.get("search", _variants()[0])))
+    solver.set_solve(cfg.get("solve", "passes"), cfg.get("solve_block", 256), cfg.get("solve_passes", 0))
     if cfg["mode"] == "graph":
         solver.use_graph = True
         solver.graph_slots = cfg["graph_slots"]
@@ -123,7 +152,10 @@ def tune(solver, state_a, state_b, dt: float, steps: int = 6, cache: Path | None
         return min(out, key=lambda r: r["ms_per_step"])
 
     search = run(search_candidates(solver.gather_block))["search"]
-    run(candidates(its, search))
+    base = run(candidates(its, search))
+    sc = solve_candidates(base, solver)
+    if sc:
+        run(sc)
     best = min(results, key=lambda r: r["ms_per_step"])
     chosen = {k: v for k, v in best.items()}
     apply(solver, chosen)

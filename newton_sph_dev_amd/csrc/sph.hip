@@ -172,6 +172,10 @@ struct SphDev {
     int* anyrb;      // [0]: some world searches this step (guards the sort and reorder)
     int* built;      // [0]: a sorted layout exists (the first step searches every world)
     int nwave;   // 4 per chunk
+    // whole-world solve: worlds in the order the workgroups take them (largest first) and
+    // the queue position (cleared by k_sph_begin)
+    const int* w_order;
+    int* wq;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2. Renumber them
@@ -446,6 +450,7 @@ __global__ void k_sph_begin(SphDev D, int W) {
         D.flags[2] = 0;
         D.bq_count[0] = 0;
         D.anyrb[0] = 0;
+        D.wq[0] = 0;
     }
     if (k < D.nwave) D.wmax[k] = 0;
 }
@@ -1014,10 +1019,7 @@ struct XA {
 //   v_adv  = v + dt (g + a_visc)
 // vadv.w carries the particle's p0 / rho^2 of the previous-pressure warm start, so that
 // the source pass fetches it with v_adv (one load per neighbour instead of three).
-__global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int warm) {
-    int w, s;
-    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
-    const PhysWorld P = D.phys[w];
+__device__ inline void advect_one(const SphDev& D, const PhysWorld& P, int s, float dt, int warm) {
     const float4 xi = D.xs[s];
     const float4 vi = D.vs[s];
     float ax = 0.f, ay = 0.f, az = 0.f, dv = 0.f;
@@ -1046,6 +1048,12 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
                             vi.z + dt * (P.g[2] + avz), pr0);
 }
 
+__global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int warm) {
+    int w, s;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
+    advect_one(D, D.phys[w], s, dt, warm);
+}
+
 // Source term and warm start:
 //   s_i = (rho0 - rho_i) - dt (sum_j m_j (v_adv_i - v_adv_j) . gradW_ij + psi v_adv_i . Gb_i)
 //   p_i = 0 | 0.5 p_prev | learned;  pr_i = p_i / rho_i^2
@@ -1054,10 +1062,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_advect(SphDev D, float dt, int wa
 // as pri here by k_sph_advect for the zero and previous-pressure warm starts, or by
 // k_sph_mlp_warm, which runs between the two passes, for the learned one), the same
 // operations, operands and order as a source pass followed by k_sph_pacc.
-__global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
-    int w, s;
-    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
-    const PhysWorld P = D.phys[w];
+__device__ inline void source_pacc_one(const SphDev& D, const PhysWorld& P, int s, float dt, int warm) {
     const float4 xi = D.xs[s];
     const float4 vi = D.vadv[s];
     const float rho = D.rho[s];
@@ -1086,6 +1091,12 @@ __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, i
     D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pri);
     const float cb = P.psi * pri;
     D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
+}
+
+__global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
+    int w, s;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
+    source_pacc_one(D, D.phys[w], s, dt, warm);
 }
 
 // Pressure acceleration  a^p_i = -( sum_j m_j (pr_i + pr_j) gradW_ij + (psi pr_i) Gb_i ).
@@ -1121,70 +1132,79 @@ __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int w, int
     *oz = -(az + cb * gb.z);
 }
 
+__device__ inline void pacc_one(const SphDev& D, const PhysWorld& P, int w, int s) {
+    float ax, ay, az;
+    press_acc(D, P, w, s, D.xs[s], &ax, &ay, &az);
+    D.ap[s] = make_float4(ax, ay, az, 0.f);
+}
+
 __global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
     int w, s;
     const bool ok = chunk_slot_sub(D, D.nsub, &w, &s);
     if (D.conv[w]) return;
     if (!ok) return;
-    const PhysWorld P = D.phys[w];
-    float ax, ay, az;
-    press_acc(D, P, w, s, D.xs[s], &ax, &ay, &az);
-    D.ap[s] = make_float4(ax, ay, az, 0.f);
+    pacc_one(D, D.phys[w], w, s);
 }
 
 // (A p)_i = dt^2 ( sum_j m_j (a^p_i - a^p_j) . gradW_ij + psi a^p_i . Gb_i )
 // e_i     = max((A p)_i - s_i, 0)   (compression of the predicted density, kg/m^3)
 // p_i    <- a_ii != 0 ? max(p_i + omega (s_i - (A p)_i) / a_ii, 0) : 0
 // partial[chunk] = sum of e_i over the chunk (FP64, fixed LDS tree).
+__device__ inline double pupdate_one(const SphDev& D, const PhysWorld& P, int s, float dt2, float omega) {
+    const float4 xi = D.xs[s];
+    const float4 ai = D.ap[s];
+    float sum = 0.f;
+    for_each_nbr<XA, 8>(
+        D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
+        [&](const XA& o, int) {
+            const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
+            const float r2 = (dx * dx + dy * dy) + dz * dz;
+            const float gf = kern_gf(P, r2);
+            sum += o.x.w * dot3(ai.x - o.a.x, ai.y - o.a.y, ai.z - o.a.z, gf * dx, gf * dy, gf * dz);
+        });
+    const float4 gb = D.gb[s];
+    sum += P.psi * dot3(ai.x, ai.y, ai.z, gb.x, gb.y, gb.z);
+    const float Ap = dt2 * sum;
+    const float si = D.src[s];
+    const double e = (double)fmaxf(Ap - si, 0.0f);
+    const float a = D.aii[s];
+    float pn = 0.f;
+    if (a != 0.0f) pn = fmaxf(D.p[s] + omega * (si - Ap) / a, 0.0f);
+    D.p[s] = pn;
+    const float rho = D.rho[s];
+    D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pn / (rho * rho));
+    return e;
+}
+
+// Sum of red[t] over each run of CHUNK threads (t = 0 .. CHUNK - 1 of the run), left in the
+// run's first entry: the fixed LDS tree of the chunk partials.  Every thread of the workgroup
+// calls it.
+__device__ inline void chunk_tree(double* red, int t) {
+    const int u = t & (CHUNK - 1);
+    __syncthreads();
+    for (int off = CHUNK / 2; off > 0; off >>= 1) {
+        if (u < off) red[t] += red[t + off];
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, float omega) {
     __shared__ double red[CHUNK];
     int w, s;
     const bool ok = chunk_slot(D, &w, &s);
     if (D.conv[w]) return;
     const int t = threadIdx.x;
-    double e = 0.0;
-    if (ok) {
-        const PhysWorld P = D.phys[w];
-        const float4 xi = D.xs[s];
-        const float4 ai = D.ap[s];
-        float sum = 0.f;
-        for_each_nbr<XA, 8>(
-            D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
-            [&](const XA& o, int) {
-                const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
-                const float r2 = (dx * dx + dy * dy) + dz * dz;
-                const float gf = kern_gf(P, r2);
-                sum += o.x.w * dot3(ai.x - o.a.x, ai.y - o.a.y, ai.z - o.a.z, gf * dx, gf * dy, gf * dz);
-            });
-        const float4 gb = D.gb[s];
-        sum += P.psi * dot3(ai.x, ai.y, ai.z, gb.x, gb.y, gb.z);
-        const float Ap = dt2 * sum;
-        const float si = D.src[s];
-        e = (double)fmaxf(Ap - si, 0.0f);
-        const float a = D.aii[s];
-        float pn = 0.f;
-        if (a != 0.0f) pn = fmaxf(D.p[s] + omega * (si - Ap) / a, 0.0f);
-        D.p[s] = pn;
-        const float rho = D.rho[s];
-        D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pn / (rho * rho));
-    }
-    red[t] = e;
-    __syncthreads();
-    for (int off = CHUNK / 2; off > 0; off >>= 1) {
-        if (t < off) red[t] += red[t + off];
-        __syncthreads();
-    }
+    red[t] = ok ? pupdate_one(D, D.phys[w], s, dt2, omega) : 0.0;
+    chunk_tree(red, t);
     if (t == 0) D.partial[xcd_swizzle(blockIdx.x, gridDim.x)] = red[0];
 }
 
 // One wave per world: err_w = (sum of its chunk partials) / (n_w rho0); count the update;
 // decide convergence. act[it & 1] counts the worlds still iterating after iteration `it`.
-__global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters, int max_iters) {
-    const int lane = threadIdx.x & 63;
-    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.act[(it + 1) & 1] = 0;
-    if (w >= W) return;
-    if (D.conv[w]) return;
+// world_reduce: one wave (lane = 0 .. 63) reduces world w's chunk partials and, in lane 0,
+// counts the update, decides convergence, stores it in conv[w] and returns it (0 = iterate on,
+// 1 = converged, 2 = stopped at max_iters).
+__device__ inline int world_reduce(const SphDev& D, int w, int lane, double tol, int min_iters, int max_iters) {
     const int first = D.w_cfirst[w], nch = D.w_nchunks[w];
     double acc = 0.0;
     int k = lane;
@@ -1207,8 +1227,19 @@ __global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters,
         if (n_it >= min_iters && err <= tol) c = 1;
         else if (n_it >= max_iters) c = 2;
         D.conv[w] = c;
-        if (c == 0) atomicAdd(&D.act[it & 1], 1);
+        return c;
     }
+    return 0;
+}
+
+__global__ void k_sph_reduce(SphDev D, int W, int it, double tol, int min_iters, int max_iters) {
+    const int lane = threadIdx.x & 63;
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.act[(it + 1) & 1] = 0;
+    if (w >= W) return;
+    if (D.conv[w]) return;
+    const int c = world_reduce(D, w, lane, tol, min_iters, max_iters);
+    if (lane == 0 && c == 0) atomicAdd(&D.act[it & 1], 1);
 }
 
 // Step summary over the worlds with fluid (one workgroup): max and sum of the iteration
@@ -1285,17 +1316,20 @@ __global__ __launch_bounds__(1024) void k_sph_report(SphDev D, int W, int act_sl
 // order: fp[s] = (f, p).  With a guard (the still-iterating count of the last iteration),
 // the step's end is enqueued before the host knows whether every world has converged and
 // does nothing unless it has (the host then iterates on and enqueues it again).
-__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const int* __restrict__ guard) {
-    if (guard && *guard != 0) return;
-    int w, s;
-    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
-    const PhysWorld P = D.phys[w];
+__device__ inline void final_one(const SphDev& D, const PhysWorld& P, int w, int s) {
     const float4 xi = D.xs[s];
     float ax, ay, az;
     press_acc(D, P, w, s, xi, &ax, &ay, &az);
     const float4 av = D.avisc[s];
     const float m = xi.w;
     D.fp[s] = make_float4(m * (av.x + ax), m * (av.y + ay), m * (av.z + az), D.p[s]);
+}
+
+__global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const int* __restrict__ guard) {
+    if (guard && *guard != 0) return;
+    int w, s;
+    if (!chunk_slot_sub(D, D.nsub, &w, &s)) return;
+    final_one(D, D.phys[w], w, s);
 }
 
 // The reference integrator (solver.py:24-46) on fluid particles, in particle order (fluid
@@ -1413,14 +1447,19 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_features(SphDev D, float dt, 
 // mean-error stop test does not see (such particles keep 0.5 p_prev).
 #define MLP_GATE_NEIGHBOURS 24.0f
 #define MLP_Y_MAX 8.0f
-__global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, float dt) {
-    __shared__ __align__(16) float xs[CHUNK * MLP_XPAD];
-    __shared__ __align__(16) __bf16 h1s[CHUNK / 64][16 * MLP_H];
-    __shared__ float ys[CHUNK];
-    int w, s;
-    const bool ok = chunk_slot(D, &w, &s);
-    const int t = threadIdx.x;
-    const PhysWorld P = D.phys[w];
+// LDS writes of a wave made visible to its other lanes.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Warm start of the 64 sorted particles of one wave (lane l holds particle s, ok when it
+// exists; every lane of the wave calls it, the MFMA tiles need them all).  xw ([64][MLP_XPAD]
+// floats), h1w ([16][MLP_H] bf16) and yw ([64] floats) are the wave's own LDS.
+__device__ inline void mlp_warm_wave(const SphDev& D, const MlpRegs& R, const PhysWorld& P, int s, bool ok,
+                                     float dt, float* xw, __bf16* h1w, float* yw) {
+    const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
     float f[MLP_XPAD];
     float punit = 0.0f;
     if (ok) {
@@ -1430,26 +1469,126 @@ __global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, floa
         for (int j = 0; j < MLP_XPAD; ++j) f[j] = 0.0f;
     }
 #pragma unroll
-    for (int j = 0; j < MLP_XPAD; ++j) xs[t * MLP_XPAD + j] = f[j];
-    __syncthreads();
-    const int wave = t >> 6, l = t & 63, g = l >> 4, c = l & 15;
-    MlpRegs R;
-    mlp_load_regs(M, R);
+    for (int j = 0; j < MLP_XPAD; ++j) xw[l * MLP_XPAD + j] = f[j];
+    wave_lds_sync();
     for (int h = 0; h < 4; ++h) {
         float y[4];
         f32x4 c2[4];
-        mlp_tile_forward(R, xs + (wave * 64 + 16 * h) * MLP_XPAD, h1s[wave], y, c2);
-        if (c < 4) ys[wave * 64 + 16 * h + 4 * g + c] = c == 0 ? y[0] : (c == 1 ? y[1] : (c == 2 ? y[2] : y[3]));
+        mlp_tile_forward(R, xw + 16 * h * MLP_XPAD, h1w, y, c2);
+        if (c < 4) yw[16 * h + 4 * g + c] = c == 0 ? y[0] : (c == 1 ? y[1] : (c == 2 ? y[2] : y[3]));
         __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
+    wave_lds_sync();
     if (ok) {
         const bool bulk = f[2] * 32.0f >= MLP_GATE_NEIGHBOURS;
-        const float y = bulk ? fminf(fmaxf(ys[t], -MLP_Y_MAX), MLP_Y_MAX) : 0.0f;
+        const float y = bulk ? fminf(fmaxf(yw[l], -MLP_Y_MAX), MLP_Y_MAX) : 0.0f;
         const float p0 = fmaxf(0.5f * D.pws[s] + y * punit, 0.0f);
         const float rho = D.rho[s];
         D.p0[s] = p0;
         reinterpret_cast<float*>(D.vadv + s)[3] = p0 / (rho * rho);  // pr_j of the source pass
+    }
+}
+
+__global__ __launch_bounds__(CHUNK) void k_sph_mlp_warm(SphDev D, MlpDev M, float dt) {
+    __shared__ __align__(16) float xs[CHUNK * MLP_XPAD];
+    __shared__ __align__(16) __bf16 h1s[CHUNK / 64][16 * MLP_H];
+    __shared__ float ys[CHUNK];
+    int w, s;
+    const bool ok = chunk_slot(D, &w, &s);
+    const int wave = threadIdx.x >> 6;
+    MlpRegs R;
+    mlp_load_regs(M, R);
+    mlp_warm_wave(D, R, D.phys[w], s, ok, dt, xs + wave * 64 * MLP_XPAD, h1s[wave], ys + wave * 64);
+}
+
+// ---- whole-world solve -------------------------------------------------------------------
+//
+// The pressure solve and forces of whole worlds in one persistent launch: each workgroup takes
+// the next world from a queue (worlds in descending fluid count, so the large ones start first
+// and the small ones fill in behind them) and runs, for that world alone, the passes the
+// per-pass schedule launches over all worlds -- advection, source + first pressure
+// acceleration, then Jacobi iterations (pressure acceleration, pressure update with the chunk
+// partials, the world's reduction and stop test) until the world has converged or reached
+// max_iters, and the forces -- with a workgroup barrier between passes.  START_SOURCE: the
+// advection (and the learned warm start between it and the source pass) ran as their own
+// launches over all worlds before this one (the network's operands would otherwise hold
+// registers through every pass).  START_RESUME: the first iterations ran as per-pass launches
+// over all worlds; this launch finishes the worlds still iterating, one at a time, and the
+// forces follow as a per-pass launch.  Worlds
+// never exchange data, so no workgroup waits for another, and iteration control needs neither
+// the host nor a grid-wide barrier.  The passes call the per-pass kernels' particle functions
+// in the same order, the chunk partials come from the same CHUNK-thread tree and the world's
+// sum from the same one-wave reduction, so every result bit equals the per-pass schedule's.
+// NB threads per workgroup (a multiple of CHUNK): a pass covers NB particles of the world at
+// a time.  Within a workgroup, global stores are visible to its other waves after a barrier.
+enum SolveStart { START_ADVECT = 0, START_SOURCE = 1, START_RESUME = 2 };
+template <int NB, int START>
+__global__ __launch_bounds__(NB) void k_sph_solve_worlds(SphDev D, float dt, float dt2, float omega, int warm,
+                                                         double tol, int min_iters, int max_iters, int W) {
+    static_assert(NB % CHUNK == 0, "whole chunks per pass");
+    __shared__ double ered[2][NB];
+    __shared__ int s_w, s_conv;
+    const int t = threadIdx.x;
+    for (;;) {
+        if (t == 0) s_w = atomicAdd(D.wq, 1);
+        __syncthreads();
+        // uniform across the workgroup: the world's constants become scalar loads
+        const int k = __builtin_amdgcn_readfirstlane(s_w);
+        if (k >= W) break;
+        const int w = D.w_order[k];
+        const int nf = D.w_nf[w];
+        // resuming: worlds that the per-pass iterations already stopped are done
+        if (nf > 0 && (START != START_RESUME || D.conv[w] == 0)) {
+            const PhysWorld P = D.phys[w];
+            const int f0 = D.w_fstart[w], c0 = D.w_cfirst[w], nr = D.w_nchunks[w] * CHUNK;
+            if constexpr (START == START_ADVECT) {
+                for (int r = t; r < nf; r += NB) advect_one(D, P, f0 + r, dt, warm);
+                __syncthreads();
+            }
+            if constexpr (START != START_RESUME) {
+                for (int r = t; r < nf; r += NB) source_pacc_one(D, P, f0 + r, dt, warm);
+                __syncthreads();
+            }
+            for (int it = START == START_RESUME ? 1 : 0;; ++it) {
+                if (it > 0) {
+                    for (int r = t; r < nf; r += NB) pacc_one(D, P, w, f0 + r);
+                    __syncthreads();
+                }
+                // Each thread updates one particle per batch of NB; the chunk tree of each
+                // CHUNK-thread run then finishes in the run's first wave: lane l adds the
+                // entries l, l + 64, l + 128, l + 192 as the tree's first two levels do (pairs
+                // 128 apart, then 64 apart) and the other six levels (32 .. 1 apart) by shifting
+                // down across lanes -- the same additions of the same operands as the LDS tree
+                // of k_sph_pupdate.  The entries alternate between two LDS buffers, so one
+                // barrier per batch suffices: a buffer is written again only after the next
+                // batch's barrier, which its readers have passed.
+                const int lane = t & 63;
+                int nb = 0;
+                for (int b = 0; b < nr; b += NB, ++nb) {
+                    const int r = b + t;
+                    double* red = ered[nb & 1];
+                    red[t] = r < nf ? pupdate_one(D, P, f0 + r, dt2, omega) : 0.0;
+                    __syncthreads();
+                    if ((t & (CHUNK - 1)) < 64 && b + (t & ~(CHUNK - 1)) < nr) {
+                        const double* e = red + (t & ~(CHUNK - 1)) + lane;
+                        double v = (e[0] + e[128]) + (e[64] + e[192]);
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+                        if (lane == 0) D.partial[c0 + (b + t) / CHUNK] = v;
+                    }
+                }
+                __syncthreads();
+                if (t < 64) {
+                    const int c = world_reduce(D, w, t, tol, min_iters, max_iters);
+                    if (t == 0) s_conv = c;
+                }
+                __syncthreads();
+                if (__builtin_amdgcn_readfirstlane(s_conv) != 0) break;
+            }
+            if constexpr (START != START_RESUME)
+                for (int r = t; r < nf; r += NB) final_one(D, P, w, f0 + r);
+        }
+        __syncthreads();  // every thread has read s_w before it is replaced
     }
 }
 
@@ -1699,17 +1838,19 @@ enum KernelId {
     KT_FINAL,
     KT_PASSTHROUGH,
     KT_INTEGRATE,
+    KT_SOLVE,  // whole-world solve (advection to forces of every world, one launch)
     KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source", "mlp_warm_start",
                                             "pressure_acc", "pressure_update", "reduce", "final",
-                                            "passthrough", "integrate"};
+                                            "passthrough", "integrate", "solve_worlds"};
 
 struct StepGraph {
     nsph_state_in in;
     nsph_state_out out;
     float dt;
     int slots;
+    int persist;          // whole-world solve captured (1 + its per-pass iterations; 0: none)
     int warm;             // warm-start mode captured
     const nsph_mlp* mlp;  // MLP whose weights the captured warm start reads
     hipGraphExec_t pro, epi;
@@ -1732,6 +1873,10 @@ struct nsph_solver {
     double skin = 0.0;
     int force_rebuild = 0;
     int gblock = CHUNK;  // gather-kernel workgroup size (64, 128 or 256)
+    int persist = 0;      // 1: whole-world solve (k_sph_solve_worlds) instead of a host-checked one
+    int pblock = CHUNK;   // its workgroup size (256, 512 or 1024)
+    int ppasses = 0;      // Jacobi iterations run per pass over all worlds before it (0: none)
+    int pgrid[3][3] = {};  // its grid per (start, block size): resident workgroups, 0 = not yet queried
     int nbr_variant = 0;  // neighbour-kernel variant (index into kNbrQuads)
     int key_bits = 0;
     size_t total_cells = 0;
@@ -1744,7 +1889,7 @@ struct nsph_solver {
     const int* d_world = nullptr;
     std::vector<int> h_nf, h_iters_last;
     // device tables
-    DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world;
+    DevBuf gw, phys, w_fstart, w_nf, w_cfirst, w_nchunks, chunk_world, w_order, wq;
     DevBuf fid, fworld, oid;
     DevBuf btable, bxs, ftable, breach, bcend, fcend;
     DevBuf keys, vals;
@@ -1837,6 +1982,8 @@ static SphDev make_dev(nsph_solver* S) {
     D.anyrb = bp<int>(S->anyrb);
     D.built = bp<int>(S->built);
     D.nwave = 4 * S->C;
+    D.w_order = bp<int>(S->w_order);
+    D.wq = bp<int>(S->wq);
     return D;
 }
 
@@ -1857,7 +2004,7 @@ static void solver_free(nsph_solver* S) {
                       &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b,
                       &S->snb, &S->scnt, &S->xbs, &S->xbk, &S->okey, &S->wflag, &S->rb, &S->anyrb,
                       &S->built, &S->kid, &S->kworld, &S->kkeys, &S->kvals, &S->kxs, &S->kvs, &S->ktable,
-                      &S->kcend, &S->kbreach, &S->kvb};
+                      &S->kcend, &S->kbreach, &S->kvb, &S->w_order, &S->wq};
     for (DevBuf* b : bufs) buf_free(*b);
     radix_free(S->rs);
     radix_free(S->rsk);
@@ -2044,6 +2191,13 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(upload(S->w_cfirst, cfirst, st));
     NSPH_TRY(upload(S->w_nchunks, nch, st));
     NSPH_TRY(upload(S->chunk_world, chunk_world, st));
+    // the whole-world solve's queue: descending fluid count, ties by world index (the host
+    // vector lives until the stream is synchronised at the end of this function)
+    std::vector<int> order(W);
+    for (int w = 0; w < W; ++w) order[w] = w;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nf[a] > nf[b]; });
+    NSPH_TRY(upload(S->w_order, order, st));
+    NSPH_TRY(buf_ensure(S->wq, 16));
     NSPH_TRY(upload(S->fid, fid, st));
     NSPH_TRY(upload(S->fworld, fworld, st));
     NSPH_TRY(upload(S->oid, oid, st));
@@ -2307,6 +2461,43 @@ static void launch_nbr_density(nsph_solver* S, bool keep, dim3 gg, dim3 gb, hipS
     else launch_nbr_variant<false>(S->nbr_variant, gg, gb, st, D, dt2);
 }
 
+// Whole-world solve: as many workgroups as the GPU holds at once (at most one per world); a
+// workgroup that finds the queue empty exits at once, so the grid never waits on itself.
+template <int NB, int START>
+static int launch_solve_t(nsph_solver* S, hipStream_t st, const SphDev& D, float dt) {
+    int& grid = S->pgrid[START][NB == 256 ? 0 : (NB == 512 ? 1 : 2)];
+    if (grid == 0) {
+        int per_cu = 0, ncu = 0;
+        NSPH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sph_solve_worlds<NB, START>, NB, 0));
+        NSPH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, S->ctx->device));
+        NSPH_CHECK(per_cu > 0 && ncu > 0, NSPH_ERR_HIP, "whole-world solve: no resident workgroup of %d threads", NB);
+        grid = per_cu * ncu;
+    }
+    const int g = std::max(1, std::min(grid, S->W));
+    const float dt2 = (float)((double)dt * (double)dt);
+    hipLaunchKernelGGL((k_sph_solve_worlds<NB, START>), dim3(g), dim3(NB), 0, st, D, dt, dt2, (float)S->omega,
+                       S->warm, S->tol, S->min_iters, S->max_iters, S->W);
+    NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+template <int NB>
+static int launch_solve_b(nsph_solver* S, hipStream_t st, const SphDev& D, float dt, int start) {
+    switch (start) {
+        case START_SOURCE: return launch_solve_t<NB, START_SOURCE>(S, st, D, dt);
+        case START_RESUME: return launch_solve_t<NB, START_RESUME>(S, st, D, dt);
+        default: return launch_solve_t<NB, START_ADVECT>(S, st, D, dt);
+    }
+}
+
+static int launch_solve(nsph_solver* S, hipStream_t st, const SphDev& D, float dt, int start) {
+    switch (S->pblock) {
+        case 512: return launch_solve_b<512>(S, st, D, dt, start);
+        case 1024: return launch_solve_b<1024>(S, st, D, dt, start);
+        default: return launch_solve_b<256>(S, st, D, dt, start);
+    }
+}
+
 static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in& in, float dt) {
     const float* q_in = in.d_q;
     const float* qd_in = in.d_qd;
@@ -2360,10 +2551,22 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
         if (S->Nb > 0)
             hipLaunchKernelGGL(k_sph_bnd_density, dim3((unsigned)std::min(S->C, 2048)), dim3(256), 0, st, D, dt2);
     });
+    if (S->persist && S->ppasses == 0 && S->warm != NSPH_WARM_MLP) {
+        // advection to forces of every world in one launch
+        TIMED(KT_SOLVE, NSPH_TRY(launch_solve(S, st, D, dt, START_ADVECT)));
+        S->last_dt = dt;
+        return NSPH_OK;
+    }
     TIMED(KT_ADVECT, hipLaunchKernelGGL(k_sph_advect, gg, gb, 0, st, D, dt, S->warm));
     if (S->warm == NSPH_WARM_MLP)
         TIMED(KT_MLP, hipLaunchKernelGGL(k_sph_mlp_warm, dim3(C), dim3(CHUNK), 0, st, D,
                                          nsph_mlp_device_view(S->mlp), dt));
+    if (S->persist && S->ppasses == 0) {
+        // source to forces of every world in one launch
+        TIMED(KT_SOLVE, NSPH_TRY(launch_solve(S, st, D, dt, START_SOURCE)));
+        S->last_dt = dt;
+        return NSPH_OK;
+    }
     TIMED(KT_SOURCE, hipLaunchKernelGGL(k_sph_source_pacc, gg, gb, 0, st, D, dt, S->warm));
     NSPH_HIP(hipGetLastError());
     S->last_dt = dt;
@@ -2411,7 +2614,8 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
     if (S->Nf > 0) {
-        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
+        if (!(S->persist && S->ppasses == 0))  // a whole-world solve from the start ends with the forces
+            TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
         TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
                                                guard, bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
                                                S->d_inv_mass, S->Nf, dt, S->v_max, in.d_f, out.d_q, out.d_qd,
@@ -2426,6 +2630,18 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const nsph_state_in&
                                  out.d_qd, out.d_f, out.d_p, S->capS > 0 ? out.d_xb : nullptr));
     }
     NSPH_HIP(hipGetLastError());
+    return NSPH_OK;
+}
+
+// After the prologue of a whole-world solve that starts with per-pass iterations: those
+// iterations, then the launch that finishes the worlds still iterating (nothing to do when the
+// prologue's persistent launch solved every world).
+static int enqueue_world_tail(nsph_solver* S, hipStream_t st, float dt) {
+    if (!S->persist || S->ppasses == 0 || S->Nf == 0) return NSPH_OK;
+    int launched = 0;
+    NSPH_TRY(enqueue_iterations(S, st, dt, 0, std::min(S->ppasses, S->max_iters), 0, &launched));
+    const SphDev D = make_dev(S);
+    TIMED(KT_SOLVE, NSPH_TRY(launch_solve(S, st, D, dt, START_RESUME)));
     return NSPH_OK;
 }
 
@@ -2455,6 +2671,15 @@ int nsph_solver_step(nsph_solver* S, const nsph_state_in* in_, const nsph_state_
     if (S->timing && S->ev_used > 4096) NSPH_TRY(flush_timing(S));
     NSPH_TRY(enqueue_prologue(S, st, in, dt));
     int launched = 0;
+    if (S->persist) {
+        // iteration control is on the device: one summary read, when a report is asked for
+        NSPH_TRY(enqueue_world_tail(S, st, dt));
+        NSPH_TRY(enqueue_epilogue(S, st, in, out, dt));
+        S->last_iters_launched = -1;
+        S->stepped = 1;
+        if (report) NSPH_TRY(collect_report(S, report));
+        return NSPH_OK;
+    }
     if (check_every == 0 || S->Nf == 0) {
         // no host checks: every iteration slot is enqueued (converged worlds skip them)
         NSPH_TRY(enqueue_iterations(S, st, dt, 0, S->max_iters, 0, &launched));
@@ -2503,7 +2728,8 @@ int nsph_solver_graph_capture(nsph_solver* S, const nsph_state_in* in, const nsp
     G.in = *in;
     G.out = *out;
     G.dt = dt;
-    G.slots = iter_slots;
+    G.slots = S->persist ? S->max_iters : iter_slots;
+    G.persist = S->persist ? 1 + S->ppasses : 0;
     G.warm = S->warm;
     G.mlp = S->warm == NSPH_WARM_MLP ? S->mlp : nullptr;
     hipStream_t cs = S->cap_stream;
@@ -2518,7 +2744,8 @@ int nsph_solver_graph_capture(nsph_solver* S, const nsph_state_in* in, const nsp
         int launched = 0;
         if (part == 0) {
             rc = enqueue_prologue(S, cs, G.in, dt);
-            if (rc == NSPH_OK) rc = enqueue_iterations(S, cs, dt, 0, iter_slots, 0, &launched);
+            if (rc == NSPH_OK && !S->persist) rc = enqueue_iterations(S, cs, dt, 0, iter_slots, 0, &launched);
+            if (rc == NSPH_OK && S->persist) rc = enqueue_world_tail(S, cs, dt);
         } else {
             rc = enqueue_epilogue(S, cs, G.in, G.out, dt);
         }
@@ -2561,10 +2788,12 @@ int nsph_solver_graph_step(nsph_solver* S, int handle, nsph_step_report* report)
     NSPH_CHECK(G.warm == S->warm && G.mlp == (S->warm == NSPH_WARM_MLP ? S->mlp : nullptr), NSPH_ERR_INVALID,
                "nsph_solver_graph_step: graph %d was captured with another warm start or MLP; capture again",
                handle);
+    NSPH_CHECK(G.persist == (S->persist ? 1 + S->ppasses : 0), NSPH_ERR_INVALID,
+               "nsph_solver_graph_step: graph %d was captured with another solve schedule; capture again", handle);
     hipStream_t st = S->ctx->stream;
     NSPH_HIP(hipGraphLaunch(G.pro, st));
-    int launched = G.slots;
-    if (G.slots < S->max_iters && S->Nf > 0) {
+    int launched = G.persist ? -1 : G.slots;
+    if (!G.persist && G.slots < S->max_iters && S->Nf > 0) {
         // the slots may not have been enough: read the still-iterating count once and finish
         // eagerly (same kernels, same order, so results equal the eager step's)
         NSPH_HIP(hipMemcpyAsync(S->h_pinned, bp<int>(S->act) + ((G.slots - 1) & 1), 4,
@@ -2725,6 +2954,19 @@ int nsph_solver_set_search(nsph_solver* S, int lds_quads) {
     }
     set_error("nsph_solver_set_search: no neighbour-kernel variant with %d LDS list quads", lds_quads);
     return NSPH_ERR_INVALID;
+}
+
+int nsph_solver_set_solve(nsph_solver* S, int mode, int block, int pass_iters) {
+    NSPH_CHECK(S != nullptr, NSPH_ERR_INVALID, "nsph_solver_set_solve: NULL solver");
+    NSPH_CHECK(mode == NSPH_SOLVE_PASSES || mode == NSPH_SOLVE_WORLDS, NSPH_ERR_INVALID,
+               "nsph_solver_set_solve: mode must be NSPH_SOLVE_PASSES or NSPH_SOLVE_WORLDS");
+    NSPH_CHECK(block == 256 || block == 512 || block == 1024, NSPH_ERR_INVALID,
+               "nsph_solver_set_solve: block must be 256, 512 or 1024");
+    NSPH_CHECK(pass_iters >= 0, NSPH_ERR_INVALID, "nsph_solver_set_solve: pass_iters must be >= 0");
+    S->persist = mode == NSPH_SOLVE_WORLDS ? 1 : 0;
+    S->pblock = block;
+    S->ppasses = pass_iters;
+    return NSPH_OK;
 }
 
 int nsph_solver_set_timing(nsph_solver* S, int enable) {

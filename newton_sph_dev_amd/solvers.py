@@ -195,6 +195,11 @@ class SolverSPH(SolverBase):
         check_every: host convergence check interval in iterations; 0 launches max_iters
             early-exit iterations with no host read.
         report: collect a step report (one host sync per step) into `last_report`.
+        solve: "passes" (one launch per pass over all worlds) or "worlds" (one persistent
+            launch in which each workgroup of `solve_block` threads solves whole worlds,
+            iterating each to its own stop test on the device; for batches of many worlds),
+            after `solve_passes` Jacobi iterations run per pass over all worlds.  Results are
+            identical; see set_solve.
 
     Walls are boundary particles.  Static collision shapes (ModelBuilder.add_shape_*) act
     through `contacts` (from Model.collide): after the SPH step each fluid particle in contact
@@ -219,7 +224,8 @@ class SolverSPH(SolverBase):
                  min_iters: int = 2, max_iters: int = 100, max_neighbors: int = 96,
                  warm_start: str = "zero", check_every: int = 1, report: bool = True,
                  skin: float | None = None, max_skin_neighbors: int | None = None,
-                 gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None):
+                 gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None,
+                 solve: str = "passes", solve_block: int = 256, solve_passes: int = 0):
         super().__init__(model)
         if model.device.type != "cuda":
             raise RuntimeError("SolverSPH runs on a ROCm GPU only (model.device is %s)" % model.device)
@@ -242,6 +248,11 @@ class SolverSPH(SolverBase):
         self.last_report: dict | None = None
         self.gather_block = int(gather_block)
         self.search_variant = self.SEARCH_VARIANTS[0]
+        if solve not in self.SOLVE_MODES:
+            raise ValueError(f"solve must be one of {self.SOLVE_MODES}")
+        self.solve = solve
+        self.solve_block = int(solve_block)
+        self.solve_passes = int(solve_passes)
         self.use_graph = bool(use_graph)
         self.graph_slots = graph_slots
         self._graphs: dict = {}
@@ -295,6 +306,7 @@ class SolverSPH(SolverBase):
         self._handle = h
         N.call("nsph_solver_set_launch", h, self.gather_block)
         N.call("nsph_solver_set_search", h, self.search_variant)
+        N.call("nsph_solver_set_solve", h, self.SOLVE_MODES.index(self.solve), self.solve_block, self.solve_passes)
         # SPH force (viscosity + pressure) of the last step, particle order; the reference's
         # solvers leave state_out.particle_f to the caller, so the step does too
         self.sph_force = torch.zeros((m.particle_count, 3), dtype=torch.float32, device=m.device)
@@ -459,6 +471,26 @@ class SolverSPH(SolverBase):
         if search is not None:
             N.call("nsph_solver_set_search", self._handle, int(search))
             self.search_variant = int(search)
+        self.clear_graphs()
+
+    # pressure-solve schedules: one launch per pass over all worlds, or one persistent launch in
+    # which each workgroup solves whole worlds (nsph_solver_set_solve)
+    SOLVE_MODES = ("passes", "worlds")
+
+    def set_solve(self, mode: str, block: int | None = None, passes: int | None = None) -> None:
+        """Schedule of the pressure solve: "passes" (one launch per pass over all worlds, one
+        host read of the still-iterating count per step) or "worlds" (one persistent launch,
+        each workgroup of `block` = 256/512/1024 threads iterating whole worlds to their own
+        stop test, largest world first; with `passes` = k > 0 the first k Jacobi iterations run
+        as per-pass launches over all worlds and the persistent launch finishes the worlds
+        still iterating).  "worlds" keeps iteration control on the device: no host read, and a
+        captured step needs no iteration slots beyond the k.  Results do not depend on it."""
+        if mode not in self.SOLVE_MODES:
+            raise ValueError(f"solve must be one of {self.SOLVE_MODES}")
+        block = self.solve_block if block is None else int(block)
+        passes = self.solve_passes if passes is None else int(passes)
+        N.call("nsph_solver_set_solve", self._handle, self.SOLVE_MODES.index(mode), block, passes)
+        self.solve, self.solve_block, self.solve_passes = mode, block, passes
         self.clear_graphs()
 
     def collect_report(self) -> dict:
