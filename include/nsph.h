@@ -285,16 +285,19 @@ typedef struct nsph_state_out {
 int nsph_solver_step(nsph_solver* s, const nsph_state_in* in, const nsph_state_out* out, float dt,
                      int check_every, nsph_step_report* report);
 /* Capture one step from state_in to state_out (fixed pointers and dt) into HIP graphs:
- * prologue + iter_slots Jacobi iterations, and the epilogue. Capture runs on a private
- * stream; nothing is allocated on the device during capture or replay. *handle names the
- * graph pair for nsph_solver_graph_step.  The warm-start mode and MLP in force at capture
- * are part of the graph. */
+ * prologue + iter_slots Jacobi iterations, and the epilogue (under NSPH_SOLVE_WORLDS the
+ * schedule's pass_iters iterations and the persistent launch instead; iter_slots is then
+ * ignored). Capture runs on a private stream; nothing is allocated on the device during
+ * capture or replay. *handle names the graph pair for nsph_solver_graph_step.  The
+ * warm-start mode, MLP and solve schedule in force at capture are part of the graph. */
 int nsph_solver_graph_capture(nsph_solver* s, const nsph_state_in* in, const nsph_state_out* out, float dt,
                               int iter_slots, int* handle);
-/* Replay a captured step on the context stream. When iter_slots < max_iters the host reads
- * the still-iterating world count once after the slots and, if needed, finishes the solve
- * with eager iterations before the epilogue; results equal nsph_solver_step's.  Fails with
- * NSPH_ERR_INVALID when the solver's warm-start mode or MLP differs from the capture's. */
+/* Replay a captured step on the context stream. Under NSPH_SOLVE_PASSES, when
+ * iter_slots < max_iters the host reads the still-iterating world count once after the slots
+ * and, if needed, finishes the solve with eager iterations before the epilogue; under
+ * NSPH_SOLVE_WORLDS nothing is read.  Results equal nsph_solver_step's.  Fails with
+ * NSPH_ERR_INVALID when the solver's warm-start mode, MLP or solve schedule differs from the
+ * capture's. */
 int nsph_solver_graph_step(nsph_solver* s, int handle, nsph_step_report* report);
 /* Free a captured step (its handle is not reused). */
 int nsph_solver_graph_destroy(nsph_solver* s, int handle);
