@@ -97,8 +97,8 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True, i
         "reduce": 0.0,
         # xs, xpr, gb, avisc, p -> fp (f, p)
         "final": 16 + 16 + 16 + 16 + 4 + 16 + L,
-        "passthrough": 0.0,
-        # inv, fid, fworld, fp, xs, vs, inv_mass -> q, qd, f, pprev (particle order)
+        # inv, fid, fworld, fp, xs, vs, inv_mass -> q, qd, f, pprev (particle order); the same
+        # launch integrates the non-fluid particles, whose bytes are not counted
         "integrate": 4 + 4 + 4 + 16 + 16 + 16 + 4 + 40,
     }
     if solve_passes > 0:

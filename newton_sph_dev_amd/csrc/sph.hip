@@ -433,9 +433,8 @@ __global__ void k_sph_decide(SphDev D, int W, int force) {
     if (r) atomicOr(D.anyrb, 1);
 }
 
-// Start of a step, one thread per world: iteration counters and flags cleared.
-__global__ void k_sph_begin(SphDev D, int W) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Start of a step (threads over worlds and waves): iteration counters and flags cleared.
+__device__ inline void begin_one(const SphDev& D, int W, int k) {
     if (k < W) {
         D.iters[k] = 0;
         D.err[k] = 0.0;
@@ -449,10 +448,15 @@ __global__ void k_sph_begin(SphDev D, int W) {
         D.flags[0] = 0;
         D.flags[2] = 0;
         D.bq_count[0] = 0;
-        D.anyrb[0] = 0;
         D.wq[0] = 0;
     }
     if (k < D.nwave) D.wmax[k] = 0;
+}
+
+__global__ void k_sph_begin(SphDev D, int W) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    begin_one(D, W, k);
+    if (k == 0) D.anyrb[0] = 0;
 }
 
 // Sort keys when some world searches anew (one thread per fluid index k): the cell of k's
@@ -481,6 +485,28 @@ __global__ void k_sph_keys(SphDev D, int n, const uint32_t* prev_keys, int2* __r
         } else {
             key = D.okey[k];
         }
+        D.okey[k] = key;
+        keys[k] = key;
+        vals[k] = (uint32_t)k;
+    }
+}
+
+// Start of a step without kept candidates, one launch (threads over worlds, waves and fluid
+// indices): the clearing of k_sph_begin, what k_sph_decide decides when no world keeps its
+// candidates (every world with fluid searches: rb = 1, anyrb = 1), and k_sph_keys for rb = 1
+// (the previous build's cell cleared, the key of the current position).
+__global__ void k_sph_begin_keys(SphDev D, int W, int n, const uint32_t* prev_keys, int2* __restrict__ table,
+                                 const float* __restrict__ q, const int* __restrict__ fid,
+                                 const int* __restrict__ fworld, const GridWorld* __restrict__ gw, uint32_t* keys,
+                                 uint32_t* __restrict__ vals) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    begin_one(D, W, k);
+    if (k < W) D.rb[k] = D.w_nf[k] > 0 ? 1 : 0;
+    if (k == 0) D.anyrb[0] = 1;
+    if (k < n) {
+        table[prev_keys[k] >> NSPH_XSUB_BITS] = make_int2(0, 0);
+        const int id = fid[k];
+        const uint32_t key = slab_key(gw[fworld[k]], q[3 * id], q[3 * id + 1], q[3 * id + 2]);
         D.okey[k] = key;
         keys[k] = key;
         vals[k] = (uint32_t)k;
@@ -1335,19 +1361,40 @@ __global__ __launch_bounds__(CHUNK) void k_sph_final(SphDev D, const int* __rest
 // The reference integrator (solver.py:24-46) on fluid particles, in particle order (fluid
 // index k, sorted position inv[k]), so that the particle-order outputs are written
 // coalesced:  v1 = v0 + (f w + g) dt;  |v1| <= v_max;  x1 = x0 + v1 dt.
-__global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const int* __restrict__ inv,
-                                const int* __restrict__ fid, const int* __restrict__ fworld,
-                                const float* __restrict__ inv_mass, int n, float dt, float v_max,
-                                const float* __restrict__ f_in, float* __restrict__ q_out,
-                                float* __restrict__ qd_out, float* __restrict__ f_out,
-                                float* __restrict__ p_out, float* __restrict__ xb_out,
-                                float* __restrict__ pprev) {
-    if (guard && *guard != 0) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int s = inv[k];
+struct StepEndArgs {
+    const int* inv;     // sorted position of fluid index k
+    const int* fid;     // particle id of fluid index k
+    const int* fworld;  // world of fluid index k
+    const int* oid;     // particle ids of the non-fluid particles
+    const float* inv_mass;
+    const int* flags;
+    const int* world;  // model.particle_world (NULL: all world 0)
+    int W, nf, no;
+    float dt, v_max;
+    const float* q_in;
+    const float* qd_in;
+    const float* f_in;  // caller's forces (NULL: none)
+    float* q_out;
+    float* qd_out;
+    float* f_out;  // SPH force out (NULL: not written)
+    float* p_out;  // state_out.sph.pressure (NULL: none)
+    float* xb_out;  // state_out.sph.search_q (NULL: none)
+    float* pprev;   // solver's copy of the pressure, fluid index order
+};
+
+__device__ inline void integrate_one(const SphDev& D, const StepEndArgs& A, int k) {
+    const int* __restrict__ fid = A.fid;
+    const float* __restrict__ inv_mass = A.inv_mass;
+    const float* __restrict__ f_in = A.f_in;
+    float* __restrict__ q_out = A.q_out;
+    float* __restrict__ qd_out = A.qd_out;
+    float* __restrict__ f_out = A.f_out;
+    float* __restrict__ p_out = A.p_out;
+    float* __restrict__ xb_out = A.xb_out;
+    const float dt = A.dt, v_max = A.v_max;
+    const int s = A.inv[k];
     const int id = fid[k];
-    const PhysWorld P = D.phys[fworld[k]];
+    const PhysWorld P = D.phys[A.fworld[k]];
     const float4 f = D.fp[s];
     const float4 xi = D.xs[s];
     const float4 v0 = D.vs[s];
@@ -1381,7 +1428,7 @@ __global__ void k_sph_integrate(SphDev D, const int* __restrict__ guard, const i
         f_out[3 * id + 1] = f.y;
         f_out[3 * id + 2] = f.z;
     }
-    pprev[k] = f.w;
+    A.pprev[k] = f.w;
     if (p_out) p_out[id] = f.w;
     if (xb_out) {
         const float4 b = D.xbk[k];
@@ -1594,16 +1641,18 @@ __global__ __launch_bounds__(NB) void k_sph_solve_worlds(SphDev D, float dt, flo
 
 // Non-fluid particles, reference integrator with the caller's force (zero if none)
 // (solver.py:24-46).
-__global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const float* __restrict__ q,
-                                  const float* __restrict__ qd, const float* __restrict__ inv_mass,
-                                  const int* __restrict__ flags, const int* __restrict__ world,
-                                  const PhysWorld* __restrict__ phys, int W, float dt, float v_max,
-                                  const float* __restrict__ f_in, float* __restrict__ q_out,
-                                  float* __restrict__ qd_out, float* __restrict__ f_out,
-                                  float* __restrict__ p_out, float* __restrict__ xb_out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int id = ids[k];
+__device__ inline void passthrough_one(const SphDev& D, const StepEndArgs& A, int k) {
+    const float* __restrict__ q = A.q_in;
+    const float* __restrict__ qd = A.qd_in;
+    const float* __restrict__ inv_mass = A.inv_mass;
+    const float* __restrict__ f_in = A.f_in;
+    float* __restrict__ q_out = A.q_out;
+    float* __restrict__ qd_out = A.qd_out;
+    float* __restrict__ f_out = A.f_out;
+    float* __restrict__ p_out = A.p_out;
+    float* __restrict__ xb_out = A.xb_out;
+    const float dt = A.dt, v_max = A.v_max;
+    const int id = A.oid[k];
     const float x0 = q[3 * id], y0 = q[3 * id + 1], z0 = q[3 * id + 2];
     if (xb_out) {  // not used by the solver: the particle's position
         xb_out[3 * id] = x0;
@@ -1616,15 +1665,15 @@ __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const floa
         f_out[3 * id + 2] = 0.f;
     }
     if (p_out) p_out[id] = 0.f;
-    if ((flags[id] & NSPH_PARTICLE_ACTIVE) == 0) {
+    if ((A.flags[id] & NSPH_PARTICLE_ACTIVE) == 0) {
         q_out[3 * id] = x0;
         q_out[3 * id + 1] = y0;
         q_out[3 * id + 2] = z0;
         return;
     }
-    int wd = world ? world[id] : 0;
-    wd = wd < 0 ? 0 : (wd >= W ? W - 1 : wd);
-    const PhysWorld& P = phys[wd];
+    int wd = A.world ? A.world[id] : 0;
+    wd = wd < 0 ? 0 : (wd >= A.W ? A.W - 1 : wd);
+    const PhysWorld& P = D.phys[wd];
     const float wi = inv_mass[id];
     const float stp = (-wi < 0.0f) ? 1.0f : 0.0f;
     const float fx = f_in ? f_in[3 * id] : 0.0f, fy = f_in ? f_in[3 * id + 1] : 0.0f,
@@ -1645,6 +1694,21 @@ __global__ void k_sph_passthrough(const int* __restrict__ ids, int n, const floa
     qd_out[3 * id] = v1x;
     qd_out[3 * id + 1] = v1y;
     qd_out[3 * id + 2] = v1z;
+}
+
+// The step's end: threads 0 .. nf - 1 integrate the fluid particles (reference integrator in
+// particle order, with the caller's force and the SPH force), threads nf .. nf + no - 1 the
+// non-fluid ones (reference integrator with the caller's force), one launch.  guard: nullptr,
+// or the still-iterating count that must be 0 for the fluid part to run; other = 0 leaves the
+// non-fluid particles out (a step whose end is enqueued a second time).
+__global__ void k_sph_step_end(SphDev D, StepEndArgs A, const int* __restrict__ guard, int other) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < A.nf) {
+        if (guard && *guard != 0) return;
+        integrate_one(D, A, k);
+    } else if (other && k < A.nf + A.no) {
+        passthrough_one(D, A, k - A.nf);
+    }
 }
 
 // Boundary build (once): keys, then reorder into bxs and mark btable.
@@ -1836,14 +1900,13 @@ enum KernelId {
     KT_PUPDATE,
     KT_REDUCE,
     KT_FINAL,
-    KT_PASSTHROUGH,
     KT_INTEGRATE,
     KT_SOLVE,  // whole-world solve (advection to forces of every world, one launch)
     KT_COUNT
 };
 static const char* kKernelNames[KT_COUNT] = {"build", "neighbors_density", "advect", "source", "mlp_warm_start",
                                             "pressure_acc", "pressure_update", "reduce", "final",
-                                            "passthrough", "integrate", "solve_worlds"};
+                                            "integrate", "solve_worlds"};
 
 struct StepGraph {
     nsph_state_in in;
@@ -2506,24 +2569,30 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     SphDev D = make_dev(S);
     EventPair* ep_build = nullptr;
     if (Nf > 0) NSPH_TRY(tic(S, KT_BUILD, &ep_build));
-    hipLaunchKernelGGL(k_sph_begin, dim3((std::max(W, D.nwave) + 255) / 256), dim3(256), 0, st, D, W);
-    if (Nf == 0) return NSPH_OK;
     // With a skin, the particles are first put in the sorted order of the last search and each
     // world's search positions are checked; the sort and reorder below then run (guarded on the
     // device) only when some world searches anew, and leave the other worlds' order unchanged.
-    // Without a skin every world searches every step.
+    // Without a skin every world searches every step, and the step starts in one launch.
     const bool keep = S->skin > 0.0;
-    if (keep) {
+    if (Nf > 0 && !keep) {
+        const int nt = std::max(std::max(W, D.nwave), Nf);
+        hipLaunchKernelGGL(k_sph_begin_keys, dim3((nt + 255) / 256), dim3(256), 0, st, D, W, Nf, S->skeys,
+                           bp<int2>(S->ftable), q_in, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
+                           bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
+        S->force_rebuild = 0;
+    } else {
+        hipLaunchKernelGGL(k_sph_begin, dim3((std::max(W, D.nwave) + 255) / 256), dim3(256), 0, st, D, W);
+        if (Nf == 0) return NSPH_OK;
         hipLaunchKernelGGL(k_sph_reorder_kept, dim3((Nf + 255) / 256), dim3(256), 0, st, D, q_in, qd_in,
                            S->d_mass, bp<int>(S->fid), S->perm, bp<float>(S->pprev), in.d_p, Nf);
         hipLaunchKernelGGL(k_sph_displace, dim3((Nf + 255) / 256), dim3(256), 0, st, D, Nf, q_in, bp<int>(S->fid),
                            bp<int>(S->fworld), in.d_xb);
+        hipLaunchKernelGGL(k_sph_decide, dim3((W + 255) / 256), dim3(256), 0, st, D, W, S->force_rebuild);
+        S->force_rebuild = 0;
+        hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, D, Nf, S->skeys,
+                           bp<int2>(S->ftable), q_in, in.d_xb, bp<int>(S->fid), bp<int>(S->fworld),
+                           bp<GridWorld>(S->gw), bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
     }
-    hipLaunchKernelGGL(k_sph_decide, dim3((W + 255) / 256), dim3(256), 0, st, D, W, S->force_rebuild);
-    S->force_rebuild = 0;
-    hipLaunchKernelGGL(k_sph_keys, dim3((Nf + 255) / 256), dim3(256), 0, st, D, Nf, S->skeys, bp<int2>(S->ftable),
-                       q_in, in.d_xb, bp<int>(S->fid), bp<int>(S->fworld), bp<GridWorld>(S->gw),
-                       bp<uint32_t>(S->keys), bp<uint32_t>(S->vals));
     NSPH_TRY(radix_sort(S->rs, bp<uint32_t>(S->keys), bp<uint32_t>(S->vals), Nf, S->key_bits,
                         st, &S->skeys, &S->perm, D.anyrb));
     hipLaunchKernelGGL(k_sph_reorder, dim3((Nf + 255) / 256), dim3(256), 0, st, D, q_in, qd_in,
@@ -2613,21 +2682,35 @@ static int enqueue_epilogue(nsph_solver* S, hipStream_t st, const nsph_state_in&
                             float dt, const int* guard = nullptr, bool boundary = true) {
     const int W = S->W, C = S->C;
     SphDev D = make_dev(S);
-    if (S->Nf > 0) {
-        if (!(S->persist && S->ppasses == 0))  // a whole-world solve from the start ends with the forces
-            TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
-        TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_integrate, dim3((S->Nf + 255) / 256), dim3(256), 0, st, D,
-                                               guard, bp<int>(S->inv), bp<int>(S->fid), bp<int>(S->fworld),
-                                               S->d_inv_mass, S->Nf, dt, S->v_max, in.d_f, out.d_q, out.d_qd,
-                                               out.d_f, out.d_p, S->capS > 0 ? out.d_xb : nullptr,
-                                               bp<float>(S->pprev)));
-    }
-    if (S->No > 0 && boundary) {
-        TIMED(KT_PASSTHROUGH,
-              hipLaunchKernelGGL(k_sph_passthrough, dim3((S->No + 255) / 256), dim3(256), 0, st,
-                                 bp<int>(S->oid), S->No, in.d_q, in.d_qd, S->d_inv_mass, S->d_flags,
-                                 S->d_world, bp<PhysWorld>(S->phys), W, dt, S->v_max, in.d_f, out.d_q,
-                                 out.d_qd, out.d_f, out.d_p, S->capS > 0 ? out.d_xb : nullptr));
+    if (S->Nf > 0 && !(S->persist && S->ppasses == 0))  // a whole-world solve from the start ends with the forces
+        TIMED(KT_FINAL, hipLaunchKernelGGL(k_sph_final, dim3(C * D.nsub), dim3(S->gblock), 0, st, D, guard));
+    const int other = S->No > 0 && boundary ? 1 : 0;
+    const int n = S->Nf + (other ? S->No : 0);
+    if (n > 0) {
+        StepEndArgs A{};
+        A.inv = bp<int>(S->inv);
+        A.fid = bp<int>(S->fid);
+        A.fworld = bp<int>(S->fworld);
+        A.oid = bp<int>(S->oid);
+        A.inv_mass = S->d_inv_mass;
+        A.flags = S->d_flags;
+        A.world = S->d_world;
+        A.W = W;
+        A.nf = S->Nf;
+        A.no = S->No;
+        A.dt = dt;
+        A.v_max = S->v_max;
+        A.q_in = in.d_q;
+        A.qd_in = in.d_qd;
+        A.f_in = in.d_f;
+        A.q_out = out.d_q;
+        A.qd_out = out.d_qd;
+        A.f_out = out.d_f;
+        A.p_out = out.d_p;
+        A.xb_out = S->capS > 0 ? out.d_xb : nullptr;
+        A.pprev = bp<float>(S->pprev);
+        TIMED(KT_INTEGRATE, hipLaunchKernelGGL(k_sph_step_end, dim3((n + 255) / 256), dim3(256), 0, st, D, A, guard,
+                                               other));
     }
     NSPH_HIP(hipGetLastError());
     return NSPH_OK;
