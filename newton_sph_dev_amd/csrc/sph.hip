@@ -981,51 +981,58 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
 }
 
 #define BND_BATCH 8
-// Boundary terms of the queued particles, continuing their compensated density sum in the
-// boundary candidate order, then the same finish as the neighbour kernel.  Grid-stride over
-// the queue (its length is known on the device only).
+// Boundary terms of queued particle i (sorted index id.x in world w), continuing its
+// compensated density sum in the boundary candidate order, then the same finish as the
+// neighbour kernel.  MOVING: moving boundary particles exist (visited after the static ones).
+template <bool MOVING>
+__device__ inline void bnd_visit(const SphDev& D, int i, int s, int w, float dt2) {
+    const GridWorld g = D.gw[w];
+    const PhysWorld P = D.phys[w];
+    const float4 xi = D.xs[s];
+    const float4 a = D.bq_a[i];
+    const float2 b = D.bq_b[i];
+    float sum = a.x, cmp = a.y;
+    float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
+    int nb = 0;
+    rows_visit<BND_BATCH>(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
+               [&](int, float dx, float dy, float dz, float r2) {
+                   float W, gf;
+                   kern_wg(P, r2, &W, &gf);
+                   neumaier(sum, cmp, P.psi * W);
+                   gbx += gf * dx;
+                   gby += gf * dy;
+                   gbz += gf * dz;
+                   mr2 = fminf(mr2, r2);
+                   ++nb;
+               });
+    // moving boundary particles after the static ones, same terms, plus v_b . gradW_ib
+    float kvb = 0.f;
+    if constexpr (MOVING)
+        rows_visit<BND_BATCH>(g, D.ktable, D.kcend, D.kxs, xi.x, xi.y, xi.z, P.H, P.H2,
+                   [&](int j, float dx, float dy, float dz, float r2) {
+                       float W, gf;
+                       kern_wg(P, r2, &W, &gf);
+                       neumaier(sum, cmp, P.psi * W);
+                       const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
+                       gbx += gx;
+                       gby += gy;
+                       gbz += gz;
+                       const float4 vb = D.kvs[j];
+                       kvb += dot3(vb.x, vb.y, vb.z, gx, gy, gz);
+                       mr2 = fminf(mr2, r2);
+                       ++nb;
+                   });
+    density_finish(D, P, s, xi.w, sum, cmp, a.z, a.w, b.x, b.y, gbx, gby, gbz, mr2, nb, dt2, kvb);
+}
+
+// Boundary terms of the queued particles in full waves.  Grid-stride over the queue (its
+// length is known on the device only).
+template <bool MOVING>
 __global__ __launch_bounds__(256) void k_sph_bnd_density(SphDev D, float dt2) {
     const int n = *D.bq_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int2 id = D.bq_id[i];
-        const int s = id.x, w = id.y;
-        const GridWorld g = D.gw[w];
-        const PhysWorld P = D.phys[w];
-        const float4 xi = D.xs[s];
-        const float4 a = D.bq_a[i];
-        const float2 b = D.bq_b[i];
-        float sum = a.x, cmp = a.y;
-        float gbx = 0.f, gby = 0.f, gbz = 0.f, mr2 = P.H2;
-        int nb = 0;
-        rows_visit<BND_BATCH>(g, D.btable, D.bcend, D.bxs, xi.x, xi.y, xi.z, P.H, P.H2,
-                   [&](int, float dx, float dy, float dz, float r2) {
-                       float W, gf;
-                       kern_wg(P, r2, &W, &gf);
-                       neumaier(sum, cmp, P.psi * W);
-                       gbx += gf * dx;
-                       gby += gf * dy;
-                       gbz += gf * dz;
-                       mr2 = fminf(mr2, r2);
-                       ++nb;
-                   });
-        // moving boundary particles after the static ones, same terms, plus v_b . gradW_ib
-        float kvb = 0.f;
-        if (D.kxs != nullptr)
-            rows_visit<BND_BATCH>(g, D.ktable, D.kcend, D.kxs, xi.x, xi.y, xi.z, P.H, P.H2,
-                       [&](int j, float dx, float dy, float dz, float r2) {
-                           float W, gf;
-                           kern_wg(P, r2, &W, &gf);
-                           neumaier(sum, cmp, P.psi * W);
-                           const float gx = gf * dx, gy = gf * dy, gz = gf * dz;
-                           gbx += gx;
-                           gby += gy;
-                           gbz += gz;
-                           const float4 vb = D.kvs[j];
-                           kvb += dot3(vb.x, vb.y, vb.z, gx, gy, gz);
-                           mr2 = fminf(mr2, r2);
-                           ++nb;
-                       });
-        density_finish(D, P, s, xi.w, sum, cmp, a.z, a.w, b.x, b.y, gbx, gby, gbz, mr2, nb, dt2, kvb);
+        bnd_visit<MOVING>(D, i, id.x, id.y, dt2);
     }
 }
 
@@ -2617,8 +2624,12 @@ static int enqueue_prologue(nsph_solver* S, hipStream_t st, const nsph_state_in&
     const dim3 gg(C * D.nsub), gb(S->gblock);
     TIMED(KT_NEIGHBORS, {
         launch_nbr_density(S, keep, gg, gb, st, D, dt2);
-        if (S->Nb > 0)
-            hipLaunchKernelGGL(k_sph_bnd_density, dim3((unsigned)std::min(S->C, 2048)), dim3(256), 0, st, D, dt2);
+        // particles near static or moving boundary particles were queued for this pass
+        if (S->Nb > 0 || S->Nk > 0) {
+            const dim3 bg((unsigned)std::min(S->C, 2048));
+            if (S->Nk > 0) hipLaunchKernelGGL(k_sph_bnd_density<true>, bg, dim3(256), 0, st, D, dt2);
+            else hipLaunchKernelGGL(k_sph_bnd_density<false>, bg, dim3(256), 0, st, D, dt2);
+        }
     });
     if (S->persist && S->ppasses == 0 && S->warm != NSPH_WARM_MLP) {
         // advection to forces of every world in one launch
