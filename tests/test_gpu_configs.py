@@ -11,8 +11,9 @@
       0.1 % mean compression; the step after the warm start equals the FP32 restatement
       started from the same warm-start pressure (the network's prediction).
   C5  1024 worlds (4096..65536 fluid particles, uniform) with the learned warm start in captured
-      (HIP graph) steps: three replays equal three eager steps bit for bit; sampled worlds
-      equal the FP32 restatement from the same warm-start pressure, bit for bit.
+      (HIP graph) steps, per pass and with the whole-world launch: three replays of each equal
+      three eager steps bit for bit; sampled worlds equal the FP32 restatement from the same
+      warm-start pressure, bit for bit.
 """
 
 from __future__ import annotations
@@ -172,25 +173,35 @@ def test_c5_batched_1024_mlp_graph():
     mlp = WarmStartMLP.load(str(MLP_PATH))
     se = SolverSPH(m, H)
     se.set_mlp(mlp)
+    # captured steps of both schedules: per pass with 3 iteration slots, and 2 per-pass
+    # iterations then the whole-world launch (no host read inside the step)
     sg = SolverSPH(m, H, use_graph=True, graph_slots=3)
     sg.set_mlp(mlp)
+    sw = SolverSPH(m, H, use_graph=True, solve="worlds", solve_block=512, solve_passes=2)
+    sw.set_mlp(mlp)
     ea, eb = m.state(), m.state()
     ga, gb = m.state(), m.state()
+    wa, wb = m.state(), m.state()
     for k in range(3):
         se.step(ea, eb, None, None, scenes.DT)
         if k == 0:
             p0_eager = _np(se.warm_pressure())
         sg.step(ga, gb, None, None, scenes.DT)
+        sw.step(wa, wb, None, None, scenes.DT)
         torch.cuda.synchronize()
         assert se.last_report["unconverged_worlds"] == 0
         assert torch.equal(eb.particle_q, gb.particle_q) and torch.equal(eb.particle_qd, gb.particle_qd), k
         assert torch.equal(eb.sph.pressure, gb.sph.pressure), k
+        assert torch.equal(eb.particle_q, wb.particle_q) and torch.equal(eb.particle_qd, wb.particle_qd), k
+        assert torch.equal(eb.sph.pressure, wb.sph.pressure), k
+        assert np.array_equal(se.world_stats()["iterations"], sw.world_stats()["iterations"]), k
         if k == 0:
             first = (eb.particle_q.clone(), eb.particle_qd.clone())
         ea, eb = eb, ea
         ga, gb = gb, ga
-    _print(f"  {se.fluid_count} fluid particles; graph replays equal eager steps")
-    del sg, ga, gb, ea, eb
+        wa, wb = wb, wa
+    _print(f"  {se.fluid_count} fluid particles; graph replays of both schedules equal eager steps")
+    del sg, sw, ga, gb, wa, wb, ea, eb
     # sampled worlds of the first step: alone (GPU, same network) and the FP32 restatement
     # from the batch's warm-start pressure; the first step is re-run to have its per-world stats
     se0 = SolverSPH(m, H)
