@@ -147,6 +147,10 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
             break
     el = time.perf_counter() - t0
     nf = solver.fluid_count
+    # one thread: the first step again
+    t1 = time.perf_counter()
+    O.step(pb, pb.q0, pb.qd0, dt=dt, precision="f32", nthreads=1)
+    el1 = time.perf_counter() - t1
     return {
         "value": nf * steps / el,
         "unit": "particle-updates/s",
@@ -156,6 +160,8 @@ def cpu_baseline(model, solver, q0, qd0, dt, budget_s: float = 20.0) -> dict:
         "sample": f"{steps} steps of the same workload from the initial state, same-order FP32 C "
                   f"restatement searching neighbours every step (OpenMP, {nt} threads on {nt} host cores, "
                   f"{cpu_model})",
+        "single_thread": {"value": nf / el1, "unit": "particle-updates/s", "cores": 1,
+                          "sample": "the first step of the same workload, one thread"},
     }
 
 
