@@ -1241,7 +1241,15 @@ __device__ inline int world_reduce(const SphDev& D, int w, int lane, double tol,
     const int first = D.w_cfirst[w], nch = D.w_nchunks[w];
     double acc = 0.0;
     int k = lane;
-    // lane sums partials lane, lane+64, ... in order; eight loads are issued before the adds
+    // lane sums partials lane, lane+64, ... in order; sixteen (then eight) loads are issued
+    // before the adds
+    for (; k + 15 * 64 < nch; k += 16 * 64) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = D.partial[first + k + u * 64];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += v[u];
+    }
     for (; k + 7 * 64 < nch; k += 8 * 64) {
         double v[8];
 #pragma unroll
