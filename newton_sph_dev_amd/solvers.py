@@ -198,8 +198,10 @@ class SolverSPH(SolverBase):
         solve: "passes" (one launch per pass over all worlds) or "worlds" (one persistent
             launch in which each workgroup of `solve_block` threads solves whole worlds,
             iterating each to its own stop test on the device; for batches of many worlds),
-            after `solve_passes` Jacobi iterations run per pass over all worlds.  Results are
-            identical; see set_solve.
+            after `solve_passes` Jacobi iterations run per pass over all worlds (default 0).
+            "auto" (the default): "worlds" after min_iters per-pass iterations for models of
+            at least AUTO_WORLDS worlds, "passes" otherwise.  Results are identical; see
+            set_solve.
 
     Walls are boundary particles.  Static collision shapes (ModelBuilder.add_shape_*) act
     through `contacts` (from Model.collide): after the SPH step each fluid particle in contact
@@ -225,7 +227,7 @@ class SolverSPH(SolverBase):
                  warm_start: str = "zero", check_every: int = 1, report: bool = True,
                  skin: float | None = None, max_skin_neighbors: int | None = None,
                  gather_block: int = 256, use_graph: bool = False, graph_slots: int | None = None,
-                 solve: str = "passes", solve_block: int = 256, solve_passes: int = 0):
+                 solve: str = "auto", solve_block: int = 512, solve_passes: int | None = None):
         super().__init__(model)
         if model.device.type != "cuda":
             raise RuntimeError("SolverSPH runs on a ROCm GPU only (model.device is %s)" % model.device)
@@ -248,11 +250,17 @@ class SolverSPH(SolverBase):
         self.last_report: dict | None = None
         self.gather_block = int(gather_block)
         self.search_variant = self.SEARCH_VARIANTS[0]
-        if solve not in self.SOLVE_MODES:
-            raise ValueError(f"solve must be one of {self.SOLVE_MODES}")
+        if solve != "auto" and solve not in self.SOLVE_MODES:
+            raise ValueError(f"solve must be 'auto' or one of {self.SOLVE_MODES}")
+        if solve == "auto":
+            # batches of many worlds: iteration control on the device, the first min_iters
+            # iterations as launches over all worlds (their L2 locality), the rest per world
+            solve = "worlds" if self.W >= self.AUTO_WORLDS else "passes"
+            if solve_passes is None:
+                solve_passes = max(int(min_iters), 1)
         self.solve = solve
         self.solve_block = int(solve_block)
-        self.solve_passes = int(solve_passes)
+        self.solve_passes = int(solve_passes or 0)
         self.use_graph = bool(use_graph)
         self.graph_slots = graph_slots
         self._graphs: dict = {}
@@ -476,6 +484,7 @@ class SolverSPH(SolverBase):
     # pressure-solve schedules: one launch per pass over all worlds, or one persistent launch in
     # which each workgroup solves whole worlds (nsph_solver_set_solve)
     SOLVE_MODES = ("passes", "worlds")
+    AUTO_WORLDS = 64  # worlds from which solve="auto" takes the whole-world schedule
 
     def set_solve(self, mode: str, block: int | None = None, passes: int | None = None) -> None:
         """Schedule of the pressure solve: "passes" (one launch per pass over all worlds, one

@@ -171,11 +171,11 @@ def test_c5_batched_1024_mlp_graph():
     b, H = scenes.build_batched(cfg)
     m = b.finalize(DEV)
     mlp = WarmStartMLP.load(str(MLP_PATH))
-    se = SolverSPH(m, H)
+    se = SolverSPH(m, H, solve="passes")
     se.set_mlp(mlp)
     # captured steps of both schedules: per pass with 3 iteration slots, and 2 per-pass
     # iterations then the whole-world launch (no host read inside the step)
-    sg = SolverSPH(m, H, use_graph=True, graph_slots=3)
+    sg = SolverSPH(m, H, use_graph=True, graph_slots=3, solve="passes")
     sg.set_mlp(mlp)
     sw = SolverSPH(m, H, use_graph=True, solve="worlds", solve_block=512, solve_passes=2)
     sw.set_mlp(mlp)
