@@ -101,6 +101,18 @@ __device__ inline float dot3(float ax, float ay, float az, float bx, float by, f
     return (ax * bx + ay * by) + az * bz;
 }
 
+// Three consecutive floats of record k (12-byte stride), one dwordx3 access.
+struct F3 {
+    float x, y, z;
+};
+__device__ inline float3 load3(const float* p, int k) {
+    const F3 v = *reinterpret_cast<const F3*>(p + 3 * (size_t)k);
+    return make_float3(v.x, v.y, v.z);
+}
+__device__ inline void store3(float* p, int k, float x, float y, float z) {
+    *reinterpret_cast<F3*>(p + 3 * (size_t)k) = F3{x, y, z};
+}
+
 struct SphDev {
     int Nf, cap;
     int nsub;  // workgroups per chunk for the gather kernels (CHUNK / gather block size)
@@ -141,7 +153,10 @@ struct SphDev {
     float4* xpr;
     float4* fp;  // (force, p) of the step's end, sorted order
     int* mixed;  // per world: 1 when its fluid particles do not all have one mass
-    float4* ap;
+    // pressure acceleration a^p of the current iteration, three floats per particle: a 12-byte
+    // record puts 10 2/3 particles in a cache line where a 16-byte one puts 8, so a wave-wide
+    // load of it touches fewer lines
+    float* a3;
     float* pws;  // warm-start input in sorted order
     float* p0;   // warm-start pressure the iterations started from, sorted order
     int4* nbr4;
@@ -1123,7 +1138,7 @@ __device__ inline void source_pacc_one(const SphDev& D, const PhysWorld& P, int 
     D.p0[s] = p0;
     D.xpr[s] = make_float4(xi.x, xi.y, xi.z, pri);
     const float cb = P.psi * pri;
-    D.ap[s] = make_float4(-(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z), 0.f);
+    store3(D.a3, s, -(ax + cb * gb.x), -(ay + cb * gb.y), -(az + cb * gb.z));
 }
 
 __global__ __launch_bounds__(CHUNK) void k_sph_source_pacc(SphDev D, float dt, int warm) {
@@ -1166,9 +1181,10 @@ __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int w, int
 }
 
 __device__ inline void pacc_one(const SphDev& D, const PhysWorld& P, int w, int s) {
+    const float4 xi = D.xs[s];
     float ax, ay, az;
-    press_acc(D, P, w, s, D.xs[s], &ax, &ay, &az);
-    D.ap[s] = make_float4(ax, ay, az, 0.f);
+    press_acc(D, P, w, s, xi, &ax, &ay, &az);
+    store3(D.a3, s, ax, ay, az);
 }
 
 __global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
@@ -1183,13 +1199,18 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pacc(SphDev D) {
 // e_i     = max((A p)_i - s_i, 0)   (compression of the predicted density, kg/m^3)
 // p_i    <- a_ii != 0 ? max(p_i + omega (s_i - (A p)_i) / a_ii, 0) : 0
 // partial[chunk] = sum of e_i over the chunk (FP64, fixed LDS tree).
-__device__ inline double pupdate_one(const SphDev& D, const PhysWorld& P, int s, float dt2, float omega) {
+struct XF3 {
+    float4 x;  // x, y, z, m
+    float3 a;  // a^p
+};
+
+__device__ inline double pupdate_one(const SphDev& D, const PhysWorld& P, int w, int s, float dt2, float omega) {
     const float4 xi = D.xs[s];
-    const float4 ai = D.ap[s];
+    const float3 ai = load3(D.a3, s);
     float sum = 0.f;
-    for_each_nbr<XA, 8>(
-        D, s, [&](int j) { return XA{D.xs[j], D.ap[j]}; },
-        [&](const XA& o, int) {
+    for_each_nbr<XF3, 8>(
+        D, s, [&](int j) { return XF3{D.xs[j], load3(D.a3, j)}; },
+        [&](const XF3& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
             const float gf = kern_gf(P, r2);
@@ -1227,7 +1248,7 @@ __global__ __launch_bounds__(CHUNK) void k_sph_pupdate(SphDev D, float dt2, floa
     const bool ok = chunk_slot(D, &w, &s);
     if (D.conv[w]) return;
     const int t = threadIdx.x;
-    red[t] = ok ? pupdate_one(D, D.phys[w], s, dt2, omega) : 0.0;
+    red[t] = ok ? pupdate_one(D, D.phys[w], w, s, dt2, omega) : 0.0;
     chunk_tree(red, t);
     if (t == 0) D.partial[xcd_swizzle(blockIdx.x, gridDim.x)] = red[0];
 }
@@ -1629,7 +1650,7 @@ __global__ __launch_bounds__(NB) void k_sph_solve_worlds(SphDev D, float dt, flo
                 for (int b = 0; b < nr; b += NB, ++nb) {
                     const int r = b + t;
                     double* red = ered[nb & 1];
-                    red[t] = r < nf ? pupdate_one(D, P, f0 + r, dt2, omega) : 0.0;
+                    red[t] = r < nf ? pupdate_one(D, P, w, f0 + r, dt2, omega) : 0.0;
                     __syncthreads();
                     if ((t & (CHUNK - 1)) < 64 && b + (t & ~(CHUNK - 1)) < nr) {
                         const double* e = red + (t & ~(CHUNK - 1)) + lane;
@@ -1980,7 +2001,7 @@ struct nsph_solver {
     uint32_t* kskeys = nullptr;
     uint32_t* kperm = nullptr;
     RadixSort rsk;
-    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, ap, pws, p0, pprev, nbr, ncnt, aux;
+    DevBuf xs, vs, rho, aii, gb, avisc, vadv, src, p, xpr, mixed, fp, inv, a3, pws, p0, pprev, nbr, ncnt, aux;
     DevBuf partial, iters, err, conv, act, flags, wmax, bq_count, bq_id, bq_a, bq_b;
     DevBuf snb, scnt, xbs, xbk, okey, wflag, rb, anyrb, built;
     int* h_pinned = nullptr;  // [0] act value, [16..] RepDev
@@ -2032,7 +2053,7 @@ static SphDev make_dev(nsph_solver* S) {
     D.xpr = bp<float4>(S->xpr);
     D.fp = bp<float4>(S->fp);
     D.mixed = bp<int>(S->mixed);
-    D.ap = bp<float4>(S->ap);
+    D.a3 = bp<float>(S->a3);
     D.pws = bp<float>(S->pws);
     D.p0 = bp<float>(S->p0);
     D.nbr4 = bp<int4>(S->nbr);
@@ -2077,7 +2098,7 @@ static void solver_free(nsph_solver* S) {
     DevBuf* bufs[] = {&S->gw, &S->phys, &S->w_fstart, &S->w_nf, &S->w_cfirst, &S->w_nchunks,
                       &S->chunk_world, &S->fid, &S->fworld, &S->oid, &S->btable, &S->bxs, &S->breach, &S->bcend, &S->fcend,
                       &S->ftable, &S->keys, &S->vals, &S->xs, &S->vs, &S->rho, &S->aii, &S->gb,
-                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->ap, &S->pws, &S->p0, &S->pprev,
+                      &S->avisc, &S->vadv, &S->src, &S->p, &S->xpr, &S->mixed, &S->fp, &S->inv, &S->a3, &S->pws, &S->p0, &S->pprev,
                       &S->nbr, &S->ncnt, &S->aux, &S->partial, &S->iters, &S->err, &S->conv, &S->act,
                       &S->flags, &S->wmax, &S->bq_count, &S->bq_id, &S->bq_a, &S->bq_b,
                       &S->snb, &S->scnt, &S->xbs, &S->xbk, &S->okey, &S->wflag, &S->rb, &S->anyrb,
@@ -2374,7 +2395,7 @@ static int solver_init(nsph_solver* S, const nsph_model_desc* m, const nsph_sph_
     NSPH_TRY(buf_ensure(S->fp, nf1 * 16));
     NSPH_TRY(buf_ensure(S->inv, nf1 * 4));
     NSPH_TRY(buf_ensure(S->mixed, ((size_t)W + 1) * 4));
-    NSPH_TRY(buf_ensure(S->ap, nf1 * 16));
+    NSPH_TRY(buf_ensure(S->a3, nf1 * 12));
     NSPH_TRY(buf_ensure(S->pws, nf1 * 4));
     NSPH_TRY(buf_ensure(S->p0, nf1 * 4));
     NSPH_TRY(buf_ensure(S->pprev, nf1 * 4));
