@@ -861,32 +861,35 @@ __global__ __launch_bounds__(CHUNK) __attribute__((amdgpu_waves_per_eu(6, 8))) v
     int4* __restrict__ out = D.nbr4 + s;
     int c = 0;
     int4 quad = make_int4(0, 0, 0, 0);
-    // Entries shift into the quad from .w, so it is complete, in slot order, when every
-    // fourth entry arrives.  The first LQ quads of a particle wait in LDS
-    // (lq[q * blockDim + t]) and are stored after the search, when the whole wave writes
-    // quad q of its 64 particles at once; storing each quad when it completes would leave
-    // lines partly written and evicted several times.
+    // The first LQ quads of a particle wait in LDS (lq[q * blockDim + t]): each entry is
+    // written to its slot there as it is accepted, and the quads are stored after the search,
+    // when the whole wave writes quad q of its 64 particles at once (storing each quad when it
+    // completes would leave lines partly written and evicted several times).  Entries past
+    // the LQ quads shift into a register quad from .w, which is stored when its fourth entry
+    // arrives.  Slots past the count hold whatever was there; readers stop at the count.
     extern __shared__ int4 lq[];
+    int* __restrict__ lqi = reinterpret_cast<int*>(lq);
     const int t = (int)threadIdx.x, bd = (int)blockDim.x;
-    auto put = [&](int q, int4 v) {
-        if (q < LQ) lq[q * bd + t] = v;
-        else out[(size_t)q * Nf] = v;
-    };
     auto append = [&](int j) {
         if (j == s) return;
-        if (c < cap) {
+        if (c < 4 * LQ) {
+            lqi[((c >> 2) * bd + t) * 4 + (c & 3)] = j;
+        } else if (c < cap) {
             quad = make_int4(quad.y, quad.z, quad.w, j);
-            if ((c & 3) == 3) put(c >> 2, quad);
+            if ((c & 3) == 3) out[(size_t)(c >> 2) * Nf] = quad;
         }
         ++c;
     };
-    // the last, partial quad, the LDS-staged quads, the count and the per-wave maximum
+    // the last, partial quad past the LDS, the LDS-staged quads, the count and the per-wave
+    // maximum
     auto finish_list = [&]() {
         const int cl = c < cap ? c : cap;
         const int u = cl & 3;  // entries of the last, partial quad, at its top
-        if (u == 1) put(cl >> 2, make_int4(quad.w, 0, 0, 0));
-        if (u == 2) put(cl >> 2, make_int4(quad.z, quad.w, 0, 0));
-        if (u == 3) put(cl >> 2, make_int4(quad.y, quad.z, quad.w, 0));
+        if (cl > 4 * LQ) {
+            if (u == 1) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.w, 0, 0, 0);
+            if (u == 2) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.z, quad.w, 0, 0);
+            if (u == 3) out[(size_t)(cl >> 2) * Nf] = make_int4(quad.y, quad.z, quad.w, 0);
+        }
         const int nq = min((cl + 3) >> 2, LQ);
         for (int q = 0; q < nq; ++q) out[(size_t)q * Nf] = lq[q * bd + t];
         D.ncnt[s] = cl;
