@@ -1435,8 +1435,10 @@ __device__ inline void integrate_one(const SphDev& D, const StepEndArgs& A, int 
     const int id = fid[k];
     const PhysWorld P = D.phys[A.fworld[k]];
     const float4 f = D.fp[s];
-    const float4 xi = D.xs[s];
-    const float4 v0 = D.vs[s];
+    // x0 and v0 from state_in in particle order (coalesced): the sorted xs[s] and vs[s] hold
+    // copies of the same values, which would be gathered through the permutation
+    const float4 xi = make_float4(A.q_in[3 * id], A.q_in[3 * id + 1], A.q_in[3 * id + 2], 0.f);
+    const float4 v0 = make_float4(A.qd_in[3 * id], A.qd_in[3 * id + 1], A.qd_in[3 * id + 2], 0.f);
     const float wi = inv_mass[id];
     // total force: the caller's (state_in.particle_f) plus the SPH force
     float tx = f.x, ty = f.y, tz = f.z;
