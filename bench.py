@@ -35,12 +35,12 @@ HBM_PEAK_GBS = 8000.0
 # What limits each kernel class, from rocprofv3 counters on MI355X (profiles/pmc_dam_break_1m.md):
 # VALU busy (SQ_ACTIVE_INST_VALU) and L1 line accesses per cycle (TCP_TOTAL_CACHE_ACCESSES).
 LIMITERS = {
-    "neighbors_density": "valu (~100 % busy at 4 cycles per wave64 instruction; TA 60 %, L1 0.51 lines/cycle)",
-    "pressure_update": "l1 line accesses (0.99 per CU cycle; TA 77 %, VALU 57 %)",
-    "advect": "l1 line accesses and valu (0.92 per CU cycle; TA 76 %, VALU 74 %)",
-    "source": "l1 line accesses (0.95 per CU cycle; TA 75 %, VALU 60 %)",
-    "pressure_acc": "l1 line accesses and valu (0.92 per CU cycle; VALU 88 %)",
-    "final": "l1 line accesses and valu (0.83 per CU cycle; VALU 80 %)",
+    "neighbors_density": "valu (106 % busy at 4 cycles per wave64 instruction; TA 71 %, L1 0.72 lines/cycle)",
+    "pressure_update": "l1 line accesses (0.98 per CU cycle; TA 69 %, VALU 74 %)",
+    "advect": "l1 line accesses and valu (0.80 per CU cycle; TA 57 %, VALU 88 %)",
+    "source": "l1 line accesses (0.96 per CU cycle; TA 65 %, VALU 81 %)",
+    "pressure_acc": "l1 line accesses and valu (0.71 per CU cycle; VALU 93 %)",
+    "final": "l1 line accesses and valu (0.65 per CU cycle; VALU 83 %)",
 }
 PROBE_STEPS = 10  # eager steps with every kernel class timed, before the timed steps
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
@@ -75,7 +75,7 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True, i
                       solve_passes: int = 0) -> dict[str, float]:
     """Compulsory bytes per fluid particle and launch, per kernel class (nbar = mean
     fluid-fluid list length; 4 B per list entry + 4 B count).  `passes` radix passes of
-    20 B each; the source pass also writes iteration 0's pressure acceleration (16 B) unless
+    20 B each; the source pass also writes iteration 0's pressure acceleration (12 B) unless
     the MLP warm start runs between them.  The whole-world solve runs, in one launch, the
     source pass (after advection, which runs as its own launch with the MLP warm start),
     `iters` pressure updates, iters - 1 pressure accelerations and the forces; after
@@ -88,18 +88,18 @@ def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True, i
         # xs, vs, pws -> avisc, vadv (v_adv, p0 / rho^2)
         "advect": 16 + 16 + 4 + 32 + L,
         # xs, vadv, gb, rho, pws -> src, p, xpr (x, p / rho^2) [, ap]
-        "source": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 16 + (16 if fused_source else 0) + L,
-        # xs, xpr, gb -> ap
-        "pressure_acc": 16 + 16 + 16 + 16 + L,
+        "source": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 16 + (12 if fused_source else 0) + L,
+        # xs, xpr, gb -> ap (three floats)
+        "pressure_acc": 16 + 16 + 16 + 12 + L,
         # xs, ap, gb, src, aii, p, rho -> p, xpr
-        "pressure_update": 16 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + 16 + L,
+        "pressure_update": 16 + 12 + 16 + 4 + 4 + 4 + 4 + 4 + 16 + L,
         "mlp_warm_start": 4 + 4 + 4 + 8 + 16 + 4 + 4 + 16 + 16 + 4,
         "reduce": 0.0,
         # xs, xpr, gb, avisc, p -> fp (f, p)
         "final": 16 + 16 + 16 + 16 + 4 + 16 + L,
-        # inv, fid, fworld, fp, xs, vs, inv_mass -> q, qd, f, pprev (particle order); the same
-        # launch integrates the non-fluid particles, whose bytes are not counted
-        "integrate": 4 + 4 + 4 + 16 + 16 + 16 + 4 + 40,
+        # inv, fid, fworld, fp, q_in, qd_in, inv_mass -> q, qd, f, pprev (particle order); the
+        # same launch copies the non-fluid particles through, whose bytes are not counted
+        "integrate": 4 + 4 + 4 + 16 + 12 + 12 + 4 + 40,
     }
     if solve_passes > 0:
         rest = max(iters - solve_passes, 0.0)
