@@ -58,7 +58,7 @@ def measured_traffic(workload: str, kernel: str):
 
 
 def sort_passes(params) -> int:
-    """Radix passes of the cell sort (8-bit digits, or 9-bit where that saves a pass): keys
+    """Radix passes of the cell sort (8-, 9- or 10-bit digits, whichever needs fewest): keys
     count the cells of every world's box (cell width w = H (1 + skin) (1 + 1/128) along x and
     y, w / 2 along z, at least 3 per axis) times 4 x-slabs per cell."""
     total = 0
@@ -68,7 +68,7 @@ def sort_passes(params) -> int:
         d = np.maximum(np.ceil((params.domain_hi[w] - params.domain_lo[w]) / widths) + 1.0, 3.0)
         total += int(np.prod(d))
     bits = max(int(np.ceil(np.log2(max(total * 4, 2)))), 1)
-    return min((bits + 7) // 8, (bits + 8) // 9)
+    return min((bits + 7) // 8, (bits + 8) // 9, (bits + 9) // 10)
 
 
 def algorithmic_bytes(nbar: float, passes: int = 3, fused_source: bool = True, iters: float = 2.0,

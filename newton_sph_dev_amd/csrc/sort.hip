@@ -1,5 +1,6 @@
-// Stable LSD radix sort of (uint32 key, uint32 value) pairs for gfx950, with 8- or 9-bit
-// digits (9 when that saves a pass: keys of 25-27 bits sort in 3 passes instead of 4).
+// Stable LSD radix sort of (uint32 key, uint32 value) pairs for gfx950, with 8-, 9- or 10-bit
+// digits (the narrowest width with the fewest passes: keys of 17-18 bits sort in 2 passes of
+// 9 bits, 19-20 bits in 2 of 10, 25-27 bits in 3 of 9, 28-30 bits in 3 of 10).
 //
 // Per pass (digit d = (key >> shift) & (NB - 1), NB = 2^digit bits):
 //   k_rs_hist    : one workgroup per tile of RS_TILE keys builds an NB-bin LDS histogram
@@ -204,17 +205,26 @@ int radix_reserve(RadixSort& rs, int n) {
     const int n_tiles = (n + RS_TILE - 1) / RS_TILE;
     NSPH_TRY(buf_ensure(rs.keys_alt, (size_t)n * 4));
     NSPH_TRY(buf_ensure(rs.vals_alt, (size_t)n * 4));
-    NSPH_TRY(buf_ensure(rs.hist, (size_t)n_tiles * 512 * 4));
-    NSPH_TRY(buf_ensure(rs.digit_tot, 512 * 4));
+    NSPH_TRY(buf_ensure(rs.hist, (size_t)n_tiles * 1024 * 4));
+    NSPH_TRY(buf_ensure(rs.digit_tot, 1024 * 4));
     rs.capacity = n;
     return NSPH_OK;
 }
 
-// Passes over `bits` key bits and the digit width (9-bit digits when they save a pass).
+// Passes over `bits` key bits and the digit width: the narrowest of 8, 9 and 10 bits that
+// needs the fewest passes (a wider digit costs more per pass: more bins to scan and clear).
 static int rs_passes(int bits, int* digit_bits) {
-    const int p8 = bits <= 0 ? 0 : (bits + 7) / 8, p9 = bits <= 0 ? 0 : (bits + 8) / 9;
-    *digit_bits = p9 < p8 ? 9 : 8;
-    return p9 < p8 ? p9 : p8;
+    if (bits <= 0) {
+        *digit_bits = 8;
+        return 0;
+    }
+    const int p8 = (bits + 7) / 8, p9 = (bits + 8) / 9, p10 = (bits + 9) / 10;
+    if (p8 <= p10) {
+        *digit_bits = 8;
+        return p8;
+    }
+    *digit_bits = p9 <= p10 ? 9 : 10;
+    return p10;
 }
 
 void radix_result(const RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits, uint32_t** out_keys,
@@ -240,7 +250,14 @@ int radix_sort(RadixSort& rs, uint32_t* keys, uint32_t* vals, int n, int bits,
     const int passes = rs_passes(bits, &db);
     for (int p = 0; p < passes; ++p) {
         const int shift = db * p;
-        if (db == 9) {
+        if (db == 10) {
+            hipLaunchKernelGGL(k_rs_hist<10>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
+                               bp<uint32_t>(rs.hist), n_tiles, guard);
+            hipLaunchKernelGGL(k_rs_scan, dim3(1024), dim3(RS_BLOCK), 0, stream,
+                               bp<uint32_t>(rs.hist), n_tiles, bp<uint32_t>(rs.digit_tot), guard);
+            hipLaunchKernelGGL(k_rs_scatter<10>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, va, kb,
+                               vb, n, shift, bp<uint32_t>(rs.hist), bp<uint32_t>(rs.digit_tot), n_tiles, guard);
+        } else if (db == 9) {
             hipLaunchKernelGGL(k_rs_hist<9>, dim3(n_tiles), dim3(RS_BLOCK), 0, stream, ka, n, shift,
                                bp<uint32_t>(rs.hist), n_tiles, guard);
             hipLaunchKernelGGL(k_rs_scan, dim3(512), dim3(RS_BLOCK), 0, stream,

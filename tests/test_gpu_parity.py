@@ -108,6 +108,29 @@ def test_hashgrid_device_query_counts():
         assert got.max() > 3
 
 
+@pytest.mark.parametrize("dims", [(16, 16, 16), (64, 64, 32), (64, 64, 128), (128, 128, 64), (128, 128, 128),
+                                  (512, 512, 256), (1024, 1024, 256)])
+def test_hashgrid_sort_every_digit_width(dims):
+    """The build's stable radix sort for cell counts of 2^12 .. 2^28 (two passes of 8-, 9- and
+    10-bit digits, three of 8, 9 and 10): the sorted order is numpy's stable argsort of the
+    cell index, i.e. by cell, and by point id inside a cell.  Points sit in the middle half of
+    their cells (radius 1), so no cell index depends on rounding; many share a cell."""
+    cells = dims[0] * dims[1] * dims[2]
+    _print(f"hashgrid sort: 100000 points, dims {dims} ({cells.bit_length() - 1}-bit keys)")
+    rng = np.random.default_rng(cells.bit_length())
+    n = 100_000
+    c = np.stack([rng.integers(0, d, n) for d in dims], axis=1)
+    c[n // 2:] = c[: n - n // 2][rng.integers(0, n - n // 2, n - n // 2)]  # shared cells
+    pts = (c + 0.25 + 0.5 * rng.random((n, 3))).astype(np.float32)
+    key = (c[:, 2] * dims[1] + c[:, 1]) * dims[0] + c[:, 0]
+    g = HashGrid(*dims, device="cuda:0")
+    g.build(torch.from_numpy(pts).cuda(), 1.0)
+    ids = g.point_ids().cpu().numpy()
+    assert np.array_equal(ids, np.argsort(key, kind="stable"))
+    del g
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("dims", [(12, 10, 14), (20, 16, 24)])
 def test_step_small_dam_break_parity(dims):
     cfg = scenes.small_dam_break(*dims)
