@@ -9,7 +9,7 @@
 // x1 = x0 + v1*dt, inactive particles keep x and leave v untouched.
 //
 // Layout (HBM, fluid particles in cell-sorted order s; one world = one contiguous run):
-//   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, rho)
+//   xs[s]   float4 (x, y, z, mass)         vs[s]   float4 (v, mass / rho)
 //   rho[s], aii[s], src[s], p[s]   float
 //   xpr[s]  float4 (x, y, z, p/rho^2)   (positions again, so that the pressure-acceleration
 //                                     gather of a world of one fluid mass loads 16 bytes)
@@ -819,7 +819,8 @@ __device__ inline void for_each_nbr(const SphDev& D, int s, Load load, Use use) 
 //            Gf = sum_j m_j gradW_ij,  S = sum_j m_j |gradW_ij|^2,  Gb = sum_b gradW_ib
 //            G = Gf + psi Gb,          a_ii = -dt^2 ((G.G + m_i S) / rho_i^2)
 // a_ii is the diagonal of the operator (A p)_i of k_sph_pupdate.  rho is also stored in
-// vs[s].w so that the viscosity pass fetches velocity and density together.
+// vs[s].w as m_i / rho_i, the factor the viscosity pass takes of each neighbour, so that it
+// fetches it with the velocity and divides once per particle instead of once per pair.
 // End of the density pass for sorted particle s: rho = (fluid + boundary compensated sum),
 // G = Gf + psi Gb, a_ii = -dt^2 ((G.G + m_i S) / rho^2), and the per-particle outputs.
 __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s, float mi, float sum, float cmp,
@@ -829,7 +830,7 @@ __device__ inline void density_finish(const SphDev& D, const PhysWorld& P, int s
     const float Gx = gfx + P.psi * gbx, Gy = gfy + P.psi * gby, Gz = gfz + P.psi * gbz;
     const float num = dot3(Gx, Gy, Gz, Gx, Gy, Gz) + mi * S;
     D.rho[s] = rho;
-    reinterpret_cast<float*>(D.vs + s)[3] = rho;
+    reinterpret_cast<float*>(D.vs + s)[3] = mi / rho;
     D.aux[s] = make_float2((float)nb, nb > 0 ? sqrtf(mr2) * P.invH : 1.0f);
     D.aii[s] = -(dt2 * (num / (rho * rho)));
     D.gb[s] = make_float4(gbx, gby, gbz, (float)nb);
@@ -1081,7 +1082,7 @@ __device__ inline void advect_one(const SphDev& D, const PhysWorld& P, int s, fl
             const float r2 = (dx * dx + dy * dy) + dz * dz;
             const float gf = kern_gf(P, r2);
             const float vx = dot3(vi.x - o.v.x, vi.y - o.v.y, vi.z - o.v.z, dx, dy, dz);
-            const float c = (o.x.w / o.v.w) * (vx / (r2 + P.eps2));
+            const float c = o.v.w * (vx / (r2 + P.eps2));  // (m_j / rho_j) (v_ij . x_ij) / (r^2 + eps^2)
             ax += c * (gf * dx);
             ay += c * (gf * dy);
             az += c * (gf * dz);
@@ -1094,7 +1095,8 @@ __device__ inline void advect_one(const SphDev& D, const PhysWorld& P, int s, fl
     const float4 gb = D.gb[s];
     dv += P.psi * (dot3(vi.x, vi.y, vi.z, gb.x, gb.y, gb.z) - (D.kvb ? D.kvb[s] : 0.f));
     D.avisc[s] = make_float4(avx, avy, avz, dv);
-    const float pr0 = warm == NSPH_WARM_PREV ? (0.5f * D.pws[s]) / (vi.w * vi.w) : 0.0f;
+    const float rho = D.rho[s];
+    const float pr0 = warm == NSPH_WARM_PREV ? (0.5f * D.pws[s]) / (rho * rho) : 0.0f;
     D.vadv[s] = make_float4(vi.x + dt * (P.g[0] + avx), vi.y + dt * (P.g[1] + avy),
                             vi.z + dt * (P.g[2] + avz), pr0);
 }
