@@ -62,6 +62,23 @@ struct PhysWorld {
     int skin_on;          // lists are kept between steps (skin > 0)
 };
 
+// sqrtf(x), correctly rounded, in fewer instructions for 2^-96 <= x < inf: the hardware root
+// s (within 1 ulp) is replaced by its lower neighbour when that one's square is not below x,
+// and by its upper neighbour when that one's square is below x (residuals x - s' s exact by
+// FMA), which leaves the root nearest sqrt(x).  sqrtf's own lowering adds to the same
+// correction a scaling for arguments below 2^-96 and a pass-through of 0 and inf; such
+// arguments (and NaN) go to sqrtf here.
+__device__ inline float sqrt_rn(float x) {
+    if (__builtin_expect((__float_as_uint(x) - 0x0F800000u) >= (0x7F800000u - 0x0F800000u), 0))
+        return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    float t = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    t = fmaf(-up, s, x) > 0.0f ? up : t;
+    return t;
+}
+
 // W(r) and (dW/dr)/r of the cubic spline for r^2 = r2 < H^2.
 __device__ inline void kern_wg(const PhysWorld& P, float r2, float* W, float* gf) {
     const float r = sqrtf(r2);
@@ -81,6 +98,15 @@ __device__ inline void kern_wg(const PhysWorld& P, float r2, float* W, float* gf
 
 __device__ inline float kern_gf(const PhysWorld& P, float r2) {
     const float r = sqrtf(r2);
+    const float q = r * P.invH;
+    if (q <= 0.5f) return P.gsig2 * (18.0f * q - 12.0f);
+    const float u = fmaxf(1.0f - q, 0.0f);
+    return -(P.gsig6 * (u * u)) / r;
+}
+
+// kern_gf with sqrt_rn (the same bits): the pressure passes' form
+__device__ inline float kern_gf_p(const PhysWorld& P, float r2) {
+    const float r = sqrt_rn(r2);
     const float q = r * P.invH;
     if (q <= 0.5f) return P.gsig2 * (18.0f * q - 12.0f);
     const float u = fmaxf(1.0f - q, 0.0f);
@@ -1162,7 +1188,7 @@ __device__ inline void press_acc(const SphDev& D, const PhysWorld& P, int w, int
     auto term = [&](float ox_, float oy_, float oz_, float mj, float prj) {
         const float dx = xi.x - ox_, dy = xi.y - oy_, dz = xi.z - oz_;
         const float r2 = (dx * dx + dy * dy) + dz * dz;
-        const float gf = kern_gf(P, r2);
+        const float gf = kern_gf_p(P, r2);
         const float c = mj * (pri + prj);
         ax += c * (gf * dx);
         ay += c * (gf * dy);
@@ -1218,7 +1244,7 @@ __device__ inline double pupdate_one(const SphDev& D, const PhysWorld& P, int w,
         [&](const XF3& o, int) {
             const float dx = xi.x - o.x.x, dy = xi.y - o.x.y, dz = xi.z - o.x.z;
             const float r2 = (dx * dx + dy * dy) + dz * dz;
-            const float gf = kern_gf(P, r2);
+            const float gf = kern_gf_p(P, r2);
             sum += o.x.w * dot3(ai.x - o.a.x, ai.y - o.a.y, ai.z - o.a.z, gf * dx, gf * dy, gf * dz);
         });
     const float4 gb = D.gb[s];
